@@ -1,0 +1,195 @@
+/*
+ * helm3d.h -- C ABI of the MI355X (gfx950) 3D frequency-domain Helmholtz
+ * library: the 27-point mixed-grid stencil with PML (forward, adjoint,
+ * derivative, complex wavenumber), its restarted-GMRES solve, mixed-precision
+ * and batched solves, and the FWI gradient accumulation.
+ *
+ * Plain C, plain pointers and sizes.  Every entry point returns a status
+ * (HELM3D_OK = 0) and never longjmps, so a MATLAB MEX gateway can map codes to
+ * its error identifiers.  The library keeps no hidden global state except the
+ * HIP runtime; contexts are independent and may be used from different host
+ * threads (one context per thread at a time).
+ *
+ * Conventions (all follow SLIM-release-apps tools/algorithms/3DFreqModeling):
+ *   grid      n = (nx, ny, nz) points INCLUDING the PML; x fastest,
+ *             idx = i + j*nx + k*nx*ny           (Helm3d_27pt.h:21-23)
+ *   npml      6 values [xlo, xhi, ylo, yhi, zlo, zhi] = MATLAB's 2x3 [lo;hi]
+ *             matrix in column-major order        (Helm3dmvp_mex.c:24, 71)
+ *   h         grid spacings [hx, hy, hz] in metres
+ *   wn        squared wavenumber on the PML grid, (2 pi f)^2 / v^2 for m/s
+ *             (param2wavenum.m:20-29)
+ *   host vectors at the drop-in boundary are split real/imaginary double
+ *             arrays, as mxGetPr/mxGetPi deliver them (Helm3dmvp_mex.c:67-98);
+ *   device vectors are interleaved complex (re, im) of double or float.
+ */
+#ifndef HELM3D_H
+#define HELM3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HELM3D_OK          0
+#define HELM3D_ERR_NSIZE   1  /* n not a 3-vector of positive integers  (MEX id ...:Helm3dmvp_mex:Nsize)   */
+#define HELM3D_ERR_XSIZE   2  /* x rows != prod(n)                        (MEX id ...:Helm3dkaczswp_mex:Xsize) */
+#define HELM3D_ERR_ARG     3  /* invalid argument                                                        */
+#define HELM3D_ERR_HIP     4  /* HIP runtime failure                                                     */
+#define HELM3D_ERR_NOMODEL 5  /* context has no model set                                                */
+#define HELM3D_ERR_NOMEM   6  /* device memory exhausted                                                 */
+#define HELM3D_ERR_NOTCONV 7  /* solve stopped at maxit above tolerance; x holds the last iterate        */
+
+/* ---- operator variants -------------------------------------------------- *
+ * The six MEX binaries of ibin/install_MEX:88-105 map to
+ *   Helm3dmvp_forw_mex        HELM3D_FORW
+ *   Helm3dmvp_adj_mex         HELM3D_ADJ
+ *   Helm3dmvp_forw_deriv_mex  HELM3D_FORW | HELM3D_DERIV
+ *   Helm3dmvp_adj_deriv_mex   HELM3D_ADJ  | HELM3D_DERIV
+ *   Helm3dmvp_forw_wnc_mex    HELM3D_FORW | HELM3D_WNC
+ *   Helm3dmvp_adj_wnc_mex     HELM3D_ADJ  | HELM3D_WNC
+ * (the dispatch of helm3d_operto27_mvp.m:22-42). */
+#define HELM3D_FORW  0
+#define HELM3D_ADJ   1
+#define HELM3D_DERIV 2
+#define HELM3D_WNC   4
+
+/* ---- precisions --------------------------------------------------------- */
+#define HELM3D_C128 0  /* complex128 vectors, FP64 wn and PML tables */
+#define HELM3D_C64  1  /* complex64 vectors, FP32 wn and PML tables  */
+
+/* =========================================================================
+ * 1. Drop-in matvec at the MEX boundary (host split-complex arrays)
+ * ========================================================================= */
+
+/* Replaces do_Hmvp (Helm3d_27pt.h:827) for one compile-time variant and the
+ * size checks of the gateway (Helm3dmvp_mex.c:75-85).  Synchronous: uploads
+ * wn and x to the current HIP device, applies the stencil in complex128 and
+ * downloads y.  y is fully overwritten (Helm3dmvp_mex.c:96-102).
+ *   wni     imaginary part of wn; read only for HELM3D_WNC (may be NULL otherwise)
+ *   n_len   number of entries of the MATLAB n argument (must be 3)
+ *   x_rows  rows of the MATLAB x argument (must equal prod(n))
+ *   xi      imaginary part of x, NULL for a real x (Helm3dmvp_mex.c:87-93)
+ *   nthreads accepted for signature compatibility; ignored on the GPU. */
+int helm3d_mvp(int variant, const double *wnr, const double *wni, const double *h,
+               const double *n, long long n_len, const double *npml,
+               double *yr, double *yi, const double *xr, const double *xi,
+               long long x_rows, int nthreads);
+
+/* do_Hmvp with its exact argument list (Helm3d_27pt.h:827), one symbol per
+ * MEX variant (ibin/install_MEX:88-105); returns a status instead of void. */
+int helm3d_do_Hmvp_forw(const double *wnr, const double *wni, const double *h, const double *n,
+                        const double *npml, double *yr, double *yi, const double *xr,
+                        const double *xi, int nthreads);
+int helm3d_do_Hmvp_adj(const double *wnr, const double *wni, const double *h, const double *n,
+                       const double *npml, double *yr, double *yi, const double *xr,
+                       const double *xi, int nthreads);
+int helm3d_do_Hmvp_forw_deriv(const double *wnr, const double *wni, const double *h,
+                              const double *n, const double *npml, double *yr, double *yi,
+                              const double *xr, const double *xi, int nthreads);
+int helm3d_do_Hmvp_adj_deriv(const double *wnr, const double *wni, const double *h,
+                             const double *n, const double *npml, double *yr, double *yi,
+                             const double *xr, const double *xi, int nthreads);
+int helm3d_do_Hmvp_forw_wnc(const double *wnr, const double *wni, const double *h,
+                            const double *n, const double *npml, double *yr, double *yi,
+                            const double *xr, const double *xi, int nthreads);
+int helm3d_do_Hmvp_adj_wnc(const double *wnr, const double *wni, const double *h,
+                           const double *n, const double *npml, double *yr, double *yi,
+                           const double *xr, const double *xi, int nthreads);
+
+/* =========================================================================
+ * 2. Device-resident operator context
+ * ========================================================================= */
+
+typedef struct helm3d_ctx helm3d_ctx;
+
+/* Create a context on HIP device `device` (-1: the current device).  Work is
+ * issued on `stream` (a hipStream_t, NULL: the default stream). */
+int helm3d_create(int device, void *stream, helm3d_ctx **out);
+int helm3d_destroy(helm3d_ctx *ctx);
+int helm3d_set_stream(helm3d_ctx *ctx, void *stream);
+
+/* Set the operator H(wn) of discrete_helmholtz.m:240-251: grid, spacing, PML
+ * counts and wn on the PML grid.  wn_dev points to DEVICE memory: N doubles
+ * when wn_complex == 0, N interleaved complex doubles when wn_complex != 0
+ * (complex frequency / WNCMPLX).  The context keeps its own FP64 and FP32
+ * copies, so the caller may free wn_dev afterwards. */
+int helm3d_set_model(helm3d_ctx *ctx, const int n[3], const double h[3], const int npml[6],
+                     const void *wn_dev, int wn_complex);
+
+/* Number of grid points of the current model (0 when none). */
+long long helm3d_numel(const helm3d_ctx *ctx);
+
+/* y = H x, H^H x (variant HELM3D_FORW / HELM3D_ADJ, optionally | HELM3D_WNC
+ * ignored: the model decides) on device vectors of the given precision.
+ * The matvec of helm3d_operto27_mvp.m:22-42 without host copies. */
+int helm3d_apply(helm3d_ctx *ctx, int variant, int precision, const void *x_dev, void *y_dev);
+
+/* y = (dH/dwn . dwn) x or its adjoint: the DERIV variants
+ * (Helm3d_27pt.h:202-206) with a caller-supplied real dwn (device, N values
+ * of the given precision) -- e.g. (2 pi f)^2 for s^2/m^2 (param2wavenum.m:31-34). */
+int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *dwn_dev,
+                       const void *x_dev, void *y_dev);
+
+/* ---- solves ------------------------------------------------------------- */
+
+/* Options of LinSolveOpts.m:55-62 / FGMRES.m:42-45 plus the mixed-precision
+ * controls.  Defaults via helm3d_solve_opts_default. */
+typedef struct {
+    double tol;         /* relative residual tolerance ||b - A x|| / ||b - A x0|| (1e-6) */
+    int    maxinnerit;  /* restart length m (20)                                       */
+    long long maxit;    /* maximum number of restart cycles (10000)                    */
+    int    use_x0;      /* 0: start from x = 0 (linearsolve.m:21-23); 1: x holds x0    */
+    /* mixed precision (helm3d_solve_mixed) */
+    double inner_tol;   /* relative tolerance of each complex64 inner solve (1e-2)     */
+    int    inner_m;     /* restart length of the inner complex64 GMRES (20)            */
+    int    inner_maxit; /* max inner iterations per outer step (200)                   */
+} helm3d_solve_opts;
+
+typedef struct {
+    long long iters;      /* Arnoldi steps (matvecs of the inner loop) of the FP64 loop  */
+    long long inner_iters;/* complex64 matvecs (mixed precision only)                    */
+    long long cycles;     /* restart cycles                                              */
+    double relres;        /* final true relative residual ||b - A x|| / ||r0||, FP64     */
+    double quasi_relres;  /* last quasi-residual (FGMRES.m:111)                          */
+    int converged;        /* relres < tol                                                */
+} helm3d_solve_info;
+
+void helm3d_solve_opts_default(helm3d_solve_opts *o);
+
+/* Restarted GMRES of FGMRES.m:33-128 with the identity preconditioner
+ * (linearsolve.m:94-97, 139-141): mode 1 solves H x = b, any other mode
+ * H^H x = b (linearsolve.m:44-48).  b, x: device complex128 vectors.
+ * hst (optional, host) receives up to hst_cap quasi-residuals (first = 1). */
+int helm3d_solve(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
+                 const helm3d_solve_opts *opts, helm3d_solve_info *info,
+                 double *hst, long long hst_cap, long long *n_hst);
+
+/* Mixed precision: FP64 flexible GMRES (FGMRES.m with a preconditioner,
+ * :96-98, 118) whose preconditioner is a complex64 GMRES solve of the same
+ * operator to inner_tol; FP64 outer residuals, FP64 Hessenberg least squares.
+ * Stops on the FP64 true residual at tol. */
+int helm3d_solve_mixed(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
+                       const helm3d_solve_opts *opts, helm3d_solve_info *info);
+
+/* ---- FWI gradient ------------------------------------------------------- */
+
+/* g[phys] += scale * Re( conj(U) .* (dH^H V) )[pml grid -> physical grid]:
+ * the gradient term sum_srcs(T(U)'*V) of PDEfunc.m:393-394 with
+ * T_adj(z,u) = conj(u).*(dH'*z) (discrete_helmholtz.m:288), dH the DERIV
+ * stencil with real dwn (device, N doubles), and comp2phys the truncation of
+ * the PML (discrete_helmholtz.m:231-233).  U, V: device complex128 vectors on
+ * the PML grid; g: device doubles on the physical grid
+ * (n - npml_lo - npml_hi per axis). */
+int helm3d_fwi_grad_accum(helm3d_ctx *ctx, const double *dwn_dev, const void *U_dev,
+                          const void *V_dev, double scale, double *g_dev);
+
+/* Library version string. */
+const char *helm3d_version(void);
+
+/* Text of the last error raised on the calling host thread ("" if none). */
+const char *helm3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HELM3D_H */

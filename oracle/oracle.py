@@ -1,0 +1,227 @@
+"""CPU oracle for the 3D Helmholtz hot path (TEST INFRASTRUCTURE).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this module.  It is the checker: nothing in the
+product package imports, links or executes anything under ``oracle/``.
+
+Contents
+  * ctypes bindings of ``liboracle.so`` (C restatement of ``do_Hmvp`` for the
+    six MEX variants, and of ``FGMRES.m``) and of ``_ref/libref_*.so`` (the
+    reference stencil header compiled in place by ``oracle/Makefile``);
+  * numpy restatements of the setup steps on the path:
+      - ``param2wavenum``   tools/algorithms/CommonFreqModeling/param2wavenum.m:20-49
+      - ``extend_model``    opExtension mode 1 (edge replication),
+                            tools/operators/misc/opExtension.m:94-96, applied per
+                            axis as in discrete_helmholtz.m:214-220,240
+      - ``truncate``        comp2phys = Ppad' (cut_pml), discrete_helmholtz.m:231-233,
+                            opExtension.m:89-90 (adjoint of zero padding)
+      - ``fwi_wavelet``     tools/algorithms/CommonFreqModeling/fwi_wavelet.m:24-28
+      - ``ls_misfit``       tools/algorithms/CommonFreqModeling/LS_misfit.m:21-26
+      - ``fwi_objective``   PDEfunc.m:324-394 (func = OBJ) with T_adj of
+                            discrete_helmholtz.m:288 and the source scaling of
+                            PDEfunc.m:328-339.
+
+Parity status of each function is stated in its docstring; the C parts are
+pinned against ``oracle/_ref`` (see ``tests/test_oracle_stencil.py``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VARIANTS = ("forw", "adj", "forw_deriv", "adj_deriv", "forw_wnc", "adj_wnc")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_HMVP_ARGS = [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+HMVP_FN = ctypes.CFUNCTYPE(None, *_HMVP_ARGS)
+
+_lib = None
+_ref = {}
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_dp)
+
+
+def lib():
+    """Load liboracle.so (built by ``make -C oracle``)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        for v in VARIANTS:
+            f = getattr(L, "oracle_Hmvp_" + v)
+            f.argtypes = _HMVP_ARGS
+            f.restype = None
+        L.oracle_pml_axis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_hconst.argtypes = [_dp, _dp]
+        L.oracle_fgmres.restype = ctypes.c_long
+        L.oracle_fgmres.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                    ctypes.c_long, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_long, ctypes.c_long,
+                                    _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                    _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _dp]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return all(os.path.exists(os.path.join(HERE, "_ref", f"libref_{v}.so")) for v in VARIANTS)
+
+
+def ref_lib(variant: str):
+    """The reference's do_Hmvp for one compile-time variant (oracle/_ref)."""
+    if variant not in _ref:
+        path = os.path.join(HERE, "_ref", f"libref_{variant}.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: build with `make -C oracle ref` where the reference tree exists")
+        L = ctypes.CDLL(path, mode=os.RTLD_LOCAL | getattr(os, "RTLD_NOW", 2))
+        L.do_Hmvp.argtypes = _HMVP_ARGS
+        L.do_Hmvp.restype = None
+        _ref[variant] = L
+    return _ref[variant]
+
+
+def variant_name(mode: int, deriv: bool, wn_complex: bool) -> str:
+    """MATLAB dispatch of helm3d_operto27_mvp.m:22-42."""
+    base = "forw" if mode == 1 else "adj"
+    if deriv:
+        return base + "_deriv"
+    return base + ("_wnc" if wn_complex else "")
+
+
+def _prep(wn, h, n, npml, x):
+    n = np.asarray(n, dtype=np.float64).reshape(3)
+    N = int(np.prod(n))
+    wn = np.asarray(wn).reshape(-1)
+    wnr = np.ascontiguousarray(wn.real, dtype=np.float64)
+    wni = np.ascontiguousarray(wn.imag if np.iscomplexobj(wn) else np.zeros_like(wnr), dtype=np.float64)
+    x = np.asarray(x).reshape(-1)
+    if x.size != N or wnr.size != N:
+        raise ValueError("x and wn must have prod(n) elements")
+    xr = np.ascontiguousarray(x.real, dtype=np.float64)
+    xi = np.ascontiguousarray(x.imag if np.iscomplexobj(x) else np.zeros_like(xr), dtype=np.float64)
+    h = np.ascontiguousarray(np.asarray(h, dtype=np.float64).reshape(3))
+    # npml is 2x3 column-major [lo;hi] per axis -> flat [xlo,xhi,ylo,yhi,zlo,zhi]
+    npml = np.asarray(npml, dtype=np.float64)
+    if npml.shape == (2, 3):
+        npml = npml.T.reshape(-1)
+    npml = np.ascontiguousarray(npml.reshape(6))
+    return wnr, wni, h, n, npml, xr, xi, N
+
+
+def hmvp(variant: str, wn, h, n, npml, x, nthreads: int = 0, use_ref: bool = False):
+    """y = H x (or variant) on split-complex arrays; returns complex128 (N,)."""
+    wnr, wni, h, n, npml, xr, xi, N = _prep(wn, h, n, npml, x)
+    yr = np.zeros(N)
+    yi = np.zeros(N)
+    nt = nthreads or os.cpu_count() or 1
+    if use_ref:
+        f = ref_lib(variant).do_Hmvp
+    else:
+        f = getattr(lib(), "oracle_Hmvp_" + variant)
+    f(_p(wnr), _p(wni), _p(h), _p(n), _p(npml), _p(yr), _p(yi), _p(xr), _p(xi), int(nt))
+    return yr + 1j * yi
+
+
+def pml_axis(n: int, nlo: int, nhi: int):
+    """xi_lo, xi_hi for 0-based indices -1..n (n+2 values each)."""
+    lo = np.zeros(2 * (n + 2))
+    hi = np.zeros(2 * (n + 2))
+    lib().oracle_pml_axis(int(n), int(nlo), int(nhi), _p(lo), _p(hi))
+    return lo[0::2] + 1j * lo[1::2], hi[0::2] + 1j * hi[1::2]
+
+
+def hconst(h):
+    out = np.zeros(8)
+    lib().oracle_hconst(_p(np.ascontiguousarray(np.asarray(h, dtype=np.float64))), _p(out))
+    keys = ("wn_coef", "cx", "cy", "cz", "xy", "xz", "yz", "w3a2")
+    return dict(zip(keys, out))
+
+
+def fgmres(variant: str, wn, h, n, npml, b, x0=None, tol=1e-6, maxinnerit=20, maxit=10000,
+           nthreads: int = 0, use_ref: bool = False, max_iters_cap: int = 0):
+    """Restarted GMRES of FGMRES.m (identity preconditioner) on the oracle or the
+    reference stencil.  Returns (x, info) with info = dict(iters, relres, hst, res)."""
+    wnr, wni, h, n, npml, br, bi, N = _prep(wn, h, n, npml, b)
+    if x0 is None:
+        xr = np.zeros(N)
+        xi = np.zeros(N)
+    else:
+        x0 = np.asarray(x0).reshape(-1)
+        xr = np.ascontiguousarray(x0.real, dtype=np.float64).copy()
+        xi = np.ascontiguousarray(x0.imag, dtype=np.float64).copy()
+    nt = nthreads or os.cpu_count() or 1
+    if use_ref:
+        fp = ctypes.cast(ref_lib(variant).do_Hmvp, ctypes.c_void_p)
+    else:
+        fp = ctypes.cast(getattr(lib(), "oracle_Hmvp_" + variant), ctypes.c_void_p)
+    cap = 200000
+    hst = np.zeros(cap)
+    res = np.zeros(cap)
+    nh = ctypes.c_long(0)
+    nr = ctypes.c_long(0)
+    rel = np.zeros(1)
+    iters = lib().oracle_fgmres(fp, _p(wnr), _p(wni), _p(h), _p(n), _p(npml), int(nt), N,
+                                _p(br), _p(bi), _p(xr), _p(xi), float(tol), int(maxinnerit),
+                                int(maxit), int(max_iters_cap), _p(hst), cap, ctypes.byref(nh),
+                                _p(res), cap, ctypes.byref(nr), _p(rel))
+    info = dict(iters=int(iters), relres=float(rel[0]), hst=hst[:nh.value].copy(),
+                res=res[:nr.value].copy())
+    return xr + 1j * xi, info
+
+
+# ---------------------------------------------------------------------------
+# numpy restatements of the setup / gradient steps
+# ---------------------------------------------------------------------------
+
+def param2wavenum(v, freq, unit):
+    """param2wavenum.m:20-49: returns (wn, dwn, ddwn)."""
+    c = (2 * np.pi * freq) ** 2
+    v = np.asarray(v, dtype=np.float64)
+    if unit == "m/s":
+        return c * v ** (-2), -2 * c * v ** (-3), 6 * c * v ** (-4)
+    if unit == "s2/m2":
+        return c * v, c * np.ones_like(v), np.zeros_like(v)
+    if unit == "s2/km2":
+        return 1e-6 * c * v, 1e-6 * c * np.ones_like(v), np.zeros_like(v)
+    raise ValueError("Unknown unit - " + unit)
+
+
+def extend_model(v, npml):
+    """Edge-replicating extension into the PML (opExtension.m:94-96 per axis,
+    composed as in discrete_helmholtz.m:214-217).  v is (nx,ny,nz) with x
+    fastest in memory order; npml is 2x3 [lo;hi] per axis."""
+    npml = np.asarray(npml).reshape(2, 3)
+    v = np.asarray(v)
+    pad = [(int(npml[0, a]), int(npml[1, a])) for a in range(3)]
+    return np.pad(v, pad, mode="edge")
+
+
+def truncate(u, npml):
+    """comp2phys with cut_pml (Ppad', discrete_helmholtz.m:231-233)."""
+    npml = np.asarray(npml).reshape(2, 3)
+    nt = u.shape
+    sl = tuple(slice(int(npml[0, a]), nt[a] - int(npml[1, a])) for a in range(3))
+    return u[sl]
+
+
+def fwi_wavelet(f, t0, f0):
+    """fwi_wavelet.m:24-28 (Ricker spectrum with firing time t0)."""
+    f = np.asarray(f, dtype=np.float64)
+    w = np.exp(-1j * 2 * np.pi * f * t0)
+    if f0:
+        w = f ** 2 * np.exp(-(f / f0) ** 2) * w
+    return w
+
+
+def ls_misfit(x, y):
+    """LS_misfit.m:21-26 (no offset mask): f = 1/2 ||x-y||^2, g = x - y."""
+    r = x - y
+    return 0.5 * float(np.vdot(r, r).real), r

@@ -1,0 +1,42 @@
+// Launch interface of the stencil and Krylov kernels (internal to the library).
+#pragma once
+
+#include <algorithm>
+
+#include "helm3d_common.hpp"
+
+namespace helm3d {
+
+// Operator variant bits; the six combinations built as MEX binaries in
+// ibin/install_MEX:88-105 are {0, ADJ} x {real wn, WNC, DERIV}.
+enum Variant : int { VAR_FORW = 0, VAR_ADJ = 1, VAR_DERIV = 2, VAR_WNC = 4 };
+
+template <typename R>
+struct StencilLaunch {
+    OpDesc<R> op;
+    const cpx<R>* x;     // input vector (device, interleaved complex)
+    cpx<R>* y;           // output: y = alpha * (H x) + beta * b
+    const cpx<R>* b;     // optional addend (nullptr: none)
+    R alpha, beta;
+    double* norm_part;   // optional: per-block partial sums of |y|^2
+    // optional FWI-gradient epilogue: instead of storing y, accumulate
+    // gg[phys] += gscale * Re(conj(gu) * y) over the physical box
+    // [p0, p0+pn) of each axis (comp2phys truncation of the PML).
+    const cpx<R>* gu;
+    double* gg;
+    double gscale;
+    int p0[3], pn[3];
+    int tiles_x, tiles_y, kchunk;
+};
+
+struct StencilGrid {
+    int tiles_x, tiles_y, kchunk, nchunks, blocks;
+};
+
+template <typename R>
+StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk);
+
+template <typename R>
+hipError_t launch_stencil(int variant, StencilLaunch<R> L, const StencilGrid& g, hipStream_t s);
+
+}  // namespace helm3d

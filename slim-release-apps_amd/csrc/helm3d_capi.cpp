@@ -1,0 +1,743 @@
+// C ABI of the MI355X Helmholtz library (declared in include/helm3d.h).
+//
+// Host side of the hot path: operator contexts (device-resident model, PML
+// tables, Krylov workspace), the drop-in matvec of the MEX boundary, and the
+// restarted GMRES driver of FGMRES.m with its FP64 Hessenberg least squares.
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "helm27.hpp"
+#include "helm3d.h"
+#include "krylov.hpp"
+
+using namespace helm3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(e_ == hipErrorOutOfMemory ? HELM3D_ERR_NOMEM : HELM3D_ERR_HIP,              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+#define CHK(expr)                      \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != HELM3D_OK) return s_; \
+    } while (0)
+
+// Grow-only device allocation.
+int ensure(void** p, size_t* cap, size_t bytes)
+{
+    if (*cap >= bytes && *p) return HELM3D_OK;
+    if (*p) {
+        HIPCHK(hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (bytes == 0) return HELM3D_OK;
+    HIPCHK(hipMalloc(p, bytes));
+    *cap = bytes;
+    return HELM3D_OK;
+}
+
+template <typename T>
+int ensure_t(T** p, size_t* cap, size_t count)
+{
+    void* q = *p;
+    int s = ensure(&q, cap, count * sizeof(T));
+    *p = static_cast<T*>(q);
+    return s;
+}
+
+// Least squares min ||beta e1 - H y|| for the leading (k+1) x k block of the
+// column-major (ldh x *) Hessenberg matrix H, by Givens rotations; returns the
+// explicit residual ||beta e1 - H y|| as FGMRES.m:109-111 forms it.
+double hess_lsq(int k, int ldh, const std::vector<std::complex<double>>& H, double beta,
+                std::vector<std::complex<double>>& y)
+{
+    using C = std::complex<double>;
+    std::vector<C> R((k + 1) * k), g(k + 1, C(0));
+    for (int c = 0; c < k; ++c)
+        for (int r = 0; r <= k; ++r) R[r + c * (k + 1)] = (r <= c + 1) ? H[r + c * ldh] : C(0);
+    g[0] = beta;
+    for (int c = 0; c < k; ++c) {
+        const C a = R[c + c * (k + 1)], b = R[c + 1 + c * (k + 1)];
+        const double den = std::sqrt(std::norm(a) + std::norm(b));
+        C cs(1), sn(0);
+        if (den != 0) {
+            cs = a / den;
+            sn = b / den;
+        }
+        for (int cc = c; cc < k; ++cc) {
+            const C u = R[c + cc * (k + 1)], v = R[c + 1 + cc * (k + 1)];
+            R[c + cc * (k + 1)] = std::conj(cs) * u + std::conj(sn) * v;
+            R[c + 1 + cc * (k + 1)] = -sn * u + cs * v;
+        }
+        const C u = g[c], v = g[c + 1];
+        g[c] = std::conj(cs) * u + std::conj(sn) * v;
+        g[c + 1] = -sn * u + cs * v;
+    }
+    y.assign(k, C(0));
+    for (int r = k - 1; r >= 0; --r) {
+        C s = g[r];
+        for (int c = r + 1; c < k; ++c) s -= R[r + c * (k + 1)] * y[c];
+        y[r] = s / R[r + r * (k + 1)];
+    }
+    double rs = 0;
+    for (int r = 0; r <= k; ++r) {
+        C s = (r == 0) ? C(beta) : C(0);
+        for (int c = 0; c < k; ++c)
+            if (r <= c + 1) s -= H[r + c * ldh] * y[c];
+        rs += std::norm(s);
+    }
+    return std::sqrt(rs);
+}
+
+template <typename R>
+struct Vecs {
+    cpx<R>* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct helm3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    bool has_model = false;
+    int n[3] = {0, 0, 0};
+    int npml[6] = {0, 0, 0, 0, 0, 0};
+    double h[3] = {0, 0, 0};
+    bool wnc = false;
+    long N = 0;
+    HConst hc{};
+
+    cpx<double>* tab64 = nullptr;
+    size_t tab64_cap = 0;
+    cpx<float>* tab32 = nullptr;
+    size_t tab32_cap = 0;
+    void* wn64 = nullptr;
+    size_t wn64_cap = 0;
+    void* wn32 = nullptr;
+    size_t wn32_cap = 0;
+    OpDesc<double> op64{};
+    OpDesc<float> op32{};
+    StencilGrid grid{};
+
+    double* part = nullptr;
+    size_t part_cap = 0;
+    double* small = nullptr;  // device: 2*kMaxBasis dots + norm
+    size_t small_cap = 0;
+    double* small_host = nullptr;  // pinned mirror
+
+    Vecs<double> V64, Z64, T64;
+    Vecs<float> V32, T32a, T32b;
+
+    ~helm3d_ctx()
+    {
+        (void)hipSetDevice(device);
+        void* bufs[] = {tab64, tab32, wn64, wn32, part, small, V64.p, Z64.p, T64.p, V32.p, T32a.p, T32b.p};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (small_host) (void)hipHostFree(small_host);
+    }
+};
+
+namespace {
+
+int bind(helm3d_ctx* c)
+{
+    if (!c) return fail(HELM3D_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    return HELM3D_OK;
+}
+
+template <typename R>
+OpDesc<R>& op_of(helm3d_ctx* c);
+template <>
+OpDesc<double>& op_of<double>(helm3d_ctx* c) { return c->op64; }
+template <>
+OpDesc<float>& op_of<float>(helm3d_ctx* c) { return c->op32; }
+
+template <typename R>
+Vecs<R>& basis_of(helm3d_ctx* c);
+template <>
+Vecs<double>& basis_of<double>(helm3d_ctx* c) { return c->V64; }
+template <>
+Vecs<float>& basis_of<float>(helm3d_ctx* c) { return c->V32; }
+
+// Leading dimension of a basis slot: N rounded up to 256 complex values so
+// every slot starts on a 4 KiB boundary.
+long slot_ld(long N) { return (N + 255) / 256 * 256; }
+
+int ensure_work(helm3d_ctx* c)
+{
+    const size_t nblk = std::max<size_t>(c->grid.blocks, kMaxPartials);
+    CHK(ensure_t(&c->part, &c->part_cap, nblk * 2 * kMaxBasis));
+    CHK(ensure_t(&c->small, &c->small_cap, 2 * kMaxBasis + 8));
+    if (!c->small_host) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->small_host), (2 * kMaxBasis + 8) * sizeof(double)));
+    return HELM3D_OK;
+}
+
+template <typename R>
+int ensure_vecs(Vecs<R>& v, long N, int slots)
+{
+    return ensure_t(&v.p, &v.cap, size_t(slot_ld(N)) * slots);
+}
+
+// Stencil launch on the context's operator; y = alpha*Hx + beta*b.
+template <typename R>
+int stencil(helm3d_ctx* c, int variant, const OpDesc<R>& op, const cpx<R>* x, cpx<R>* y, R alpha,
+            const cpx<R>* b, R beta, double* norm_part)
+{
+    StencilLaunch<R> L{};
+    L.op = op;
+    L.x = x;
+    L.y = y;
+    L.b = b;
+    L.alpha = alpha;
+    L.beta = beta;
+    L.norm_part = norm_part;
+    HIPCHK(launch_stencil<R>(variant, L, c->grid, c->stream));
+    return HELM3D_OK;
+}
+
+int model_variant(const helm3d_ctx* c, int mode_adj)
+{
+    return (mode_adj ? VAR_ADJ : VAR_FORW) | (c->wnc ? VAR_WNC : 0);
+}
+
+// Reduce `nq` partial columns into c->small[off..off+nq) and (optionally)
+// fetch small[0..nfetch) to the pinned host mirror.
+int reduce_to_small(helm3d_ctx* c, int nblk, int stride, int nq, int off)
+{
+    HIPCHK(launch_reduce(c->part, nblk, stride, nq, c->small + off, c->stream));
+    return HELM3D_OK;
+}
+int fetch_small(helm3d_ctx* c, int count)
+{
+    HIPCHK(hipMemcpyAsync(c->small_host, c->small, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HELM3D_OK;
+}
+
+// Flexible preconditioner hook: z = M^{-1} (s * v_raw)
+struct Precond {
+    virtual ~Precond() = default;
+    virtual int apply(const cpx<double>* v_raw, double s, cpx<double>* z) = 0;
+};
+
+struct GmresResult {
+    long iters = 0;
+    long cycles = 0;
+    double relres = 1.0;
+    double quasi = 1.0;
+    bool converged = false;
+};
+
+// Restarted GMRES of FGMRES.m:33-128 on precision R.  The basis is kept
+// unnormalised (slot c raw, scale sig[c]); P (R = double only) makes it the
+// flexible variant of FGMRES.m:96-98,118 with Z stored true-scaled.
+template <typename R>
+int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
+                 long maxit, long iter_cap, Precond* P, GmresResult* res, std::vector<double>* hst)
+{
+    using C = std::complex<double>;
+    if (m < 1 || m > kMaxBasis - 1) return fail(HELM3D_ERR_ARG, "maxinnerit must be in [1, 31]");
+    const long N = c->N;
+    const long ld = slot_ld(N);
+    OpDesc<R>& op = op_of<R>(c);
+    Vecs<R>& Vb = basis_of<R>(c);
+    CHK(ensure_vecs(Vb, N, m + 1));
+    CHK(ensure_work(c));
+    cpx<R>* V = Vb.p;
+    cpx<double>* Z = nullptr;
+    if (P) {
+        CHK(ensure_vecs(c->Z64, N, m));
+        Z = c->Z64.p;
+    }
+    const int nblk = krylov_blocks(N);
+    auto slot = [&](int s) { return V + size_t(s) * ld; };
+
+    // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62)
+    if (use_x0) {
+        CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
+        CHK(reduce_to_small(c, c->grid.blocks, 1, 1, 0));
+    } else {
+        HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, slot(0), N, false, c->stream)));
+        HIPCHK((launch_axpby<R, R>(0.0, 0.0, static_cast<const cpx<R>*>(nullptr), 0.0, 0.0, x, N, false, c->stream)));
+        HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
+        CHK(reduce_to_small(c, nblk, 1, 1, 0));
+    }
+    CHK(fetch_small(c, 1));
+    const double normr0 = std::sqrt(c->small_host[0]);
+    double hlast = 1.0;
+    if (hst) hst->push_back(1.0);
+    if (normr0 == 0.0) {
+        // zero residual: the solution is x (opAbstractMatrix.m:118 returns zeros for b = 0)
+        res->relres = 0.0;
+        res->quasi = 0.0;
+        res->converged = true;
+        return HELM3D_OK;
+    }
+    double beta = normr0;
+    long it = 1, inner = 0;
+    std::vector<C> H(size_t(m + 1) * m), y;
+    std::vector<double> sig(m + 1, 0.0);
+    bool capped = false;
+
+    while (hlast > tol && (it / m) < maxit) {
+        std::fill(H.begin(), H.end(), C(0));
+        sig[0] = 1.0 / beta;
+        int k = 1;
+        while (k <= m && hlast > tol) {
+            // w = A v_k (or A z_k) into slot k
+            if (P) {
+                if constexpr (std::is_same<R, double>::value) {
+                    CHK(P->apply(slot(k - 1), sig[k - 1], Z + size_t(k - 1) * ld));
+                    CHK(stencil<R>(c, variant, op, Z + size_t(k - 1) * ld, slot(k), R(1), nullptr, R(0), nullptr));
+                }
+            } else {
+                CHK(stencil<R>(c, variant, op, slot(k - 1), slot(k), R(sig[k - 1]), nullptr, R(0), nullptr));
+            }
+            // classical Gram-Schmidt, FGMRES.m:100-103
+            HIPCHK(launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
+            CHK(reduce_to_small(c, nblk, 2 * kMaxBasis, 2 * k, 0));
+            CoefPack scl{};
+            for (int q = 0; q < k; ++q) scl.v[q] = sig[q] * sig[q];
+            HIPCHK(launch_update<R>(V, ld, k, c->small, scl, slot(k), N, c->part, nblk, c->stream));
+            CHK(reduce_to_small(c, nblk, 1, 1, 2 * kMaxBasis));
+            // fetch d (2k values) and ||w||^2
+            HIPCHK(hipMemcpyAsync(c->small_host, c->small, 2 * k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(c->small_host + 2 * kMaxBasis, c->small + 2 * kMaxBasis, sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int q = 0; q < k; ++q)
+                H[q + (k - 1) * (m + 1)] = sig[q] * C(c->small_host[2 * q], c->small_host[2 * q + 1]);
+            const double hn = std::sqrt(c->small_host[2 * kMaxBasis]);
+            H[k + (k - 1) * (m + 1)] = hn;
+            sig[k] = 1.0 / hn;
+            hlast = hess_lsq(k, m + 1, H, beta, y) / normr0;
+            if (hst) hst->push_back(hlast);
+            ++k;
+            ++it;
+            ++inner;
+            if (iter_cap > 0 && inner >= iter_cap) {
+                capped = true;
+                break;
+            }
+        }
+        // x += V y (or Z y), FGMRES.m:115-119
+        CoefPack a{};
+        for (int q = 0; q < k - 1; ++q) {
+            const C v = P ? y[q] : y[q] * sig[q];
+            a.v[2 * q] = v.real();
+            a.v[2 * q + 1] = v.imag();
+        }
+        if (P) {
+            if constexpr (std::is_same<R, double>::value) HIPCHK(launch_combine<double>(Z, ld, k - 1, a, x, N, nblk, c->stream));
+        } else {
+            HIPCHK(launch_combine<R>(V, ld, k - 1, a, x, N, nblk, c->stream));
+        }
+        // true residual r = b - A x into slot 0, FGMRES.m:120-124
+        CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
+        CHK(reduce_to_small(c, c->grid.blocks, 1, 1, 0));
+        CHK(fetch_small(c, 1));
+        const double rn = std::sqrt(c->small_host[0]);
+        res->relres = rn / normr0;
+        ++res->cycles;
+        if (res->relres < tol) break;
+        beta = rn;
+        if (capped) break;
+    }
+    res->iters = inner;
+    res->quasi = hlast;
+    res->converged = res->relres < tol;
+    return HELM3D_OK;
+}
+
+// complex64 inner solve as the flexible preconditioner of the FP64 outer loop
+struct MixedPrecond : Precond {
+    helm3d_ctx* c;
+    int variant;
+    double inner_tol;
+    int inner_m;
+    long inner_cap;
+    long inner_total = 0;
+    int apply(const cpx<double>* v_raw, double s, cpx<double>* z) override
+    {
+        const long N = c->N;
+        // T32a = c64(s * v), solve A t = T32a in complex64, z = c128(t)
+        HIPCHK((launch_axpby<double, float>(s, 0.0, v_raw, 0.0, 0.0, c->T32a.p, N, false, c->stream)));
+        GmresResult r;
+        CHK(gmres_engine<float>(c, variant, c->T32a.p, c->T32b.p, false, inner_tol, inner_m, 1L << 40, inner_cap,
+                                nullptr, &r, nullptr));
+        inner_total += r.iters;
+        HIPCHK((launch_axpby<float, double>(1.0, 0.0, c->T32b.p, 0.0, 0.0, z, N, false, c->stream)));
+        return HELM3D_OK;
+    }
+};
+
+int check_model(helm3d_ctx* c)
+{
+    CHK(bind(c));
+    if (!c->has_model) return fail(HELM3D_ERR_NOMODEL, "no model set");
+    return HELM3D_OK;
+}
+
+std::mutex g_dropin_mu;
+helm3d_ctx* g_dropin[64] = {nullptr};
+
+}  // namespace
+
+extern "C" {
+
+const char* helm3d_version(void) { return "helm3d-mi355x 0.1"; }
+
+const char* helm3d_last_error(void) { return g_err.c_str(); }
+
+int helm3d_create(int device, void* stream, helm3d_ctx** out)
+{
+    if (!out) return fail(HELM3D_ERR_ARG, "null out");
+    *out = nullptr;
+    int dev = device;
+    if (dev < 0) HIPCHK(hipGetDevice(&dev));
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (dev >= ndev) return fail(HELM3D_ERR_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(dev));
+    helm3d_ctx* c = new helm3d_ctx();
+    c->device = dev;
+    c->stream = static_cast<hipStream_t>(stream);
+    *out = c;
+    return HELM3D_OK;
+}
+
+int helm3d_destroy(helm3d_ctx* ctx)
+{
+    if (!ctx) return HELM3D_OK;
+    delete ctx;
+    return HELM3D_OK;
+}
+
+int helm3d_set_stream(helm3d_ctx* ctx, void* stream)
+{
+    if (!ctx) return fail(HELM3D_ERR_ARG, "null context");
+    ctx->stream = static_cast<hipStream_t>(stream);
+    return HELM3D_OK;
+}
+
+long long helm3d_numel(const helm3d_ctx* ctx) { return (ctx && ctx->has_model) ? ctx->N : 0; }
+
+int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int npml[6], const void* wn_dev,
+                     int wn_complex)
+{
+    CHK(bind(c));
+    if (!n || !h || !npml || !wn_dev) return fail(HELM3D_ERR_ARG, "null argument");
+    for (int a = 0; a < 3; ++a)
+        if (n[a] < 1) return fail(HELM3D_ERR_NSIZE, "n must hold 3 positive integers");
+    for (int a = 0; a < 6; ++a)
+        if (npml[a] < 0) return fail(HELM3D_ERR_ARG, "npml must be non-negative");
+    const long N = long(n[0]) * n[1] * n[2];
+    if (N >= (1L << 40)) return fail(HELM3D_ERR_ARG, "grid too large");
+    std::memcpy(c->n, n, sizeof(c->n));
+    std::memcpy(c->npml, npml, sizeof(c->npml));
+    std::memcpy(c->h, h, sizeof(c->h));
+    c->N = N;
+    c->wnc = wn_complex != 0;
+    c->hc = make_hconst(h);
+
+    // PML tables: 6 arrays of n_a + 2 complex values
+    const size_t len = size_t(n[0] + 2) * 2 + size_t(n[1] + 2) * 2 + size_t(n[2] + 2) * 2;
+    std::vector<cpx<double>> t64(len);
+    std::vector<cpx<float>> t32(len);
+    size_t off[6], pos = 0;
+    for (int a = 0; a < 3; ++a) {
+        const PmlAxis ax = make_pml_axis(n[a], npml[2 * a], npml[2 * a + 1]);
+        off[2 * a] = pos;
+        for (int t = 0; t < n[a] + 2; ++t, ++pos) t64[pos] = cpx<double>{ax.lo[t].real(), ax.lo[t].imag()};
+        off[2 * a + 1] = pos;
+        for (int t = 0; t < n[a] + 2; ++t, ++pos) t64[pos] = cpx<double>{ax.hi[t].real(), ax.hi[t].imag()};
+    }
+    for (size_t q = 0; q < len; ++q) t32[q] = cpx<float>{float(t64[q].re), float(t64[q].im)};
+    CHK(ensure_t(&c->tab64, &c->tab64_cap, len));
+    CHK(ensure_t(&c->tab32, &c->tab32_cap, len));
+    HIPCHK(hipMemcpyAsync(c->tab64, t64.data(), len * sizeof(cpx<double>), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tab32, t32.data(), len * sizeof(cpx<float>), hipMemcpyHostToDevice, c->stream));
+
+    // wn copies (FP64 as given, FP32 for the complex64 path)
+    const size_t esz = c->wnc ? 2 : 1;
+    CHK(ensure(&c->wn64, &c->wn64_cap, size_t(N) * esz * sizeof(double)));
+    CHK(ensure(&c->wn32, &c->wn32_cap, size_t(N) * esz * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(c->wn64, wn_dev, size_t(N) * esz * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (c->wnc)
+        HIPCHK((launch_axpby<double, float>(1.0, 0.0, static_cast<const cpx<double>*>(c->wn64), 0.0, 0.0,
+                                            static_cast<cpx<float>*>(c->wn32), N, false, c->stream)));
+    else
+        HIPCHK(launch_cast_real(static_cast<const double*>(c->wn64), static_cast<float*>(c->wn32), N, c->stream));
+
+    auto fill = [&](auto& op, auto* tab, auto* wn) {
+        using R = std::remove_reference_t<decltype(op.wn_const)>;
+        op.nx = n[0];
+        op.ny = n[1];
+        op.nz = n[2];
+        op.xlo = tab + off[0];
+        op.xhi = tab + off[1];
+        op.ylo = tab + off[2];
+        op.yhi = tab + off[3];
+        op.zlo = tab + off[4];
+        op.zhi = tab + off[5];
+        if (c->wnc) {
+            op.wn = nullptr;
+            op.wnc = reinterpret_cast<const cpx<R>*>(wn);
+        } else {
+            op.wn = reinterpret_cast<const R*>(wn);
+            op.wnc = nullptr;
+        }
+        op.wn_const = R(0);
+        for (int a = 0; a < 3; ++a) op.c[a] = R(c->hc.c[a]);
+        op.xy = R(c->hc.xy);
+        op.xz = R(c->hc.xz);
+        op.yz = R(c->hc.yz);
+        op.w3a2 = R(c->hc.w3a2);
+        op.wn_coef = R(c->hc.wn_coef);
+        op.wm2 = R(kWM2);
+        op.wm3 = R(kWM3);
+        op.wm4 = R(kWM4);
+    };
+    fill(c->op64, c->tab64, c->wn64);
+    fill(c->op32, c->tab32, c->wn32);
+    c->grid = stencil_grid<double>(n[0], n[1], n[2], 0);
+    CHK(ensure_work(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_model = true;
+    return HELM3D_OK;
+}
+
+int helm3d_apply(helm3d_ctx* c, int variant, int precision, const void* x, void* y)
+{
+    CHK(check_model(c));
+    if (!x || !y) return fail(HELM3D_ERR_ARG, "null vector");
+    const int v = model_variant(c, variant & HELM3D_ADJ);
+    if (precision == HELM3D_C128)
+        return stencil<double>(c, v, c->op64, static_cast<const cpx<double>*>(x), static_cast<cpx<double>*>(y), 1.0,
+                               nullptr, 0.0, nullptr);
+    if (precision == HELM3D_C64)
+        return stencil<float>(c, v, c->op32, static_cast<const cpx<float>*>(x), static_cast<cpx<float>*>(y), 1.0f,
+                              nullptr, 0.0f, nullptr);
+    return fail(HELM3D_ERR_ARG, "bad precision");
+}
+
+int helm3d_apply_deriv(helm3d_ctx* c, int adjoint, int precision, const void* dwn, const void* x, void* y)
+{
+    CHK(check_model(c));
+    if (!x || !y || !dwn) return fail(HELM3D_ERR_ARG, "null argument");
+    const int v = (adjoint ? VAR_ADJ : VAR_FORW) | VAR_DERIV;
+    if (precision == HELM3D_C128) {
+        OpDesc<double> op = c->op64;
+        op.wn = static_cast<const double*>(dwn);
+        op.wnc = nullptr;
+        return stencil<double>(c, v, op, static_cast<const cpx<double>*>(x), static_cast<cpx<double>*>(y), 1.0,
+                               nullptr, 0.0, nullptr);
+    }
+    if (precision == HELM3D_C64) {
+        OpDesc<float> op = c->op32;
+        op.wn = static_cast<const float*>(dwn);
+        op.wnc = nullptr;
+        return stencil<float>(c, v, op, static_cast<const cpx<float>*>(x), static_cast<cpx<float>*>(y), 1.0f,
+                              nullptr, 0.0f, nullptr);
+    }
+    return fail(HELM3D_ERR_ARG, "bad precision");
+}
+
+void helm3d_solve_opts_default(helm3d_solve_opts* o)
+{
+    if (!o) return;
+    o->tol = 1e-6;
+    o->maxinnerit = 20;
+    o->maxit = 10000;
+    o->use_x0 = 0;
+    o->inner_tol = 1e-2;
+    o->inner_m = 20;
+    o->inner_maxit = 200;
+}
+
+int helm3d_solve(helm3d_ctx* c, int mode, const void* b, void* x, const helm3d_solve_opts* opts,
+                 helm3d_solve_info* info, double* hst, long long hst_cap, long long* n_hst)
+{
+    CHK(check_model(c));
+    if (!b || !x) return fail(HELM3D_ERR_ARG, "null vector");
+    helm3d_solve_opts o;
+    helm3d_solve_opts_default(&o);
+    if (opts) o = *opts;
+    std::vector<double> h;
+    GmresResult r;
+    CHK(gmres_engine<double>(c, model_variant(c, mode != 1), static_cast<const cpx<double>*>(b),
+                             static_cast<cpx<double>*>(x), o.use_x0 != 0, o.tol, o.maxinnerit, long(o.maxit), 0,
+                             nullptr, &r, hst ? &h : nullptr));
+    if (hst) {
+        const long long nh = std::min<long long>(hst_cap, (long long)h.size());
+        std::memcpy(hst, h.data(), size_t(nh) * sizeof(double));
+        if (n_hst) *n_hst = nh;
+    }
+    if (info) {
+        info->iters = r.iters;
+        info->inner_iters = 0;
+        info->cycles = r.cycles;
+        info->relres = r.relres;
+        info->quasi_relres = r.quasi;
+        info->converged = r.converged;
+    }
+    return r.converged ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
+}
+
+int helm3d_solve_mixed(helm3d_ctx* c, int mode, const void* b, void* x, const helm3d_solve_opts* opts,
+                       helm3d_solve_info* info)
+{
+    CHK(check_model(c));
+    if (!b || !x) return fail(HELM3D_ERR_ARG, "null vector");
+    helm3d_solve_opts o;
+    helm3d_solve_opts_default(&o);
+    if (opts) o = *opts;
+    CHK(ensure_vecs(c->T32a, c->N, 1));
+    CHK(ensure_vecs(c->T32b, c->N, 1));
+    MixedPrecond P;
+    P.c = c;
+    P.variant = model_variant(c, mode != 1);
+    P.inner_tol = o.inner_tol;
+    P.inner_m = o.inner_m;
+    P.inner_cap = o.inner_maxit;
+    GmresResult r;
+    CHK(gmres_engine<double>(c, P.variant, static_cast<const cpx<double>*>(b), static_cast<cpx<double>*>(x),
+                             o.use_x0 != 0, o.tol, o.maxinnerit, long(o.maxit), 0, &P, &r, nullptr));
+    if (info) {
+        info->iters = r.iters;
+        info->inner_iters = P.inner_total;
+        info->cycles = r.cycles;
+        info->relres = r.relres;
+        info->quasi_relres = r.quasi;
+        info->converged = r.converged;
+    }
+    return r.converged ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
+}
+
+int helm3d_fwi_grad_accum(helm3d_ctx* c, const double* dwn, const void* U, const void* V, double scale,
+                          double* g)
+{
+    CHK(check_model(c));
+    if (!U || !V || !g) return fail(HELM3D_ERR_ARG, "null argument");
+    StencilLaunch<double> L{};
+    L.op = c->op64;
+    L.op.wn = dwn;
+    L.op.wnc = nullptr;
+    L.op.wn_const = 1.0;
+    L.x = static_cast<const cpx<double>*>(V);
+    L.y = nullptr;
+    L.alpha = 1.0;
+    L.gu = static_cast<const cpx<double>*>(U);
+    L.gg = g;
+    L.gscale = scale;
+    for (int a = 0; a < 3; ++a) {
+        L.p0[a] = c->npml[2 * a];
+        L.pn[a] = c->n[a] - c->npml[2 * a] - c->npml[2 * a + 1];
+        if (L.pn[a] < 1) return fail(HELM3D_ERR_ARG, "PML covers the whole axis");
+    }
+    HIPCHK(launch_stencil<double>(VAR_ADJ | VAR_DERIV, L, c->grid, c->stream));
+    return HELM3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Drop-in matvec at the MEX boundary
+// ---------------------------------------------------------------------------
+
+int helm3d_mvp(int variant, const double* wnr, const double* wni, const double* h, const double* n,
+               long long n_len, const double* npml, double* yr, double* yi, const double* xr, const double* xi,
+               long long x_rows, int nthreads)
+{
+    (void)nthreads;
+    if (n_len != 3 || !n) return fail(HELM3D_ERR_NSIZE, "n must be a 3-length vector");
+    int ni[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!(n[a] >= 1) || n[a] != std::floor(n[a]) || n[a] > 2147483647.0)
+            return fail(HELM3D_ERR_NSIZE, "n must be a 3-length vector");
+        ni[a] = int(n[a]);
+    }
+    const long long N = (long long)ni[0] * ni[1] * ni[2];
+    if (x_rows != N) return fail(HELM3D_ERR_XSIZE, "x must have nx*ny*nz elements");
+    if (variant < 0 || variant > 7) return fail(HELM3D_ERR_ARG, "bad variant");
+    const bool wnc = variant & HELM3D_WNC;
+    if (!wnr || !h || !npml || !yr || !yi || !xr || (wnc && !wni)) return fail(HELM3D_ERR_ARG, "null argument");
+    int np[6];
+    for (int a = 0; a < 6; ++a) {
+        if (!(npml[a] >= 0)) return fail(HELM3D_ERR_ARG, "npml must be non-negative");
+        np[a] = int(npml[a]);
+    }
+
+    std::lock_guard<std::mutex> lock(g_dropin_mu);
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(HELM3D_ERR_ARG, "device index out of range");
+    if (!g_dropin[dev]) CHK(helm3d_create(dev, nullptr, &g_dropin[dev]));
+    helm3d_ctx* c = g_dropin[dev];
+    // staging: wn (interleaved when complex), x, y on device
+    Vecs<double>& st = c->T64;
+    CHK(ensure_vecs(st, N, 3));
+    const long ld = slot_ld(N);
+    double* d_planar = reinterpret_cast<double*>(st.p);           // 2N doubles of slot 0
+    cpx<double>* d_x = st.p + ld;                                 // slot 1
+    cpx<double>* d_y = st.p + 2 * ld;                             // slot 2
+    // wn -> slot 0 (real: first N doubles; complex: interleaved)
+    if (wnc) {
+        HIPCHK(hipMemcpyAsync(d_planar, wnr, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(d_planar + N, wni, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_interleave(d_planar, d_planar + N, d_y, N, c->stream));  // y slot as scratch
+        CHK(helm3d_set_model(c, ni, h, np, d_y, 1));
+    } else {
+        HIPCHK(hipMemcpyAsync(d_planar, wnr, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        CHK(helm3d_set_model(c, ni, h, np, d_planar, 0));
+    }
+    // x planar -> interleaved (imaginary zeros for a real x, Helm3dmvp_mex.c:87-93)
+    HIPCHK(hipMemcpyAsync(d_planar, xr, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (xi) HIPCHK(hipMemcpyAsync(d_planar + N, xi, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(launch_interleave(d_planar, xi ? d_planar + N : nullptr, d_x, N, c->stream));
+    const int v = (variant & (HELM3D_ADJ | HELM3D_DERIV)) | (wnc ? VAR_WNC : 0);
+    CHK(stencil<double>(c, v, c->op64, d_x, d_y, 1.0, nullptr, 0.0, nullptr));
+    HIPCHK(launch_deinterleave(d_y, d_planar, d_planar + N, N, c->stream));
+    HIPCHK(hipMemcpyAsync(yr, d_planar, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(yi, d_planar + N, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HELM3D_OK;
+}
+
+#define HELM3D_DO_HMVP(name, var)                                                                          \
+    int name(const double* wnr, const double* wni, const double* h, const double* n, const double* npml,   \
+             double* yr, double* yi, const double* xr, const double* xi, int nthreads)                     \
+    {                                                                                                      \
+        if (!n) return fail(HELM3D_ERR_NSIZE, "n must be a 3-length vector");                             \
+        const long long N = (long long)n[0] * (long long)n[1] * (long long)n[2];                           \
+        return helm3d_mvp(var, wnr, wni, h, n, 3, npml, yr, yi, xr, xi, N, nthreads);                      \
+    }
+HELM3D_DO_HMVP(helm3d_do_Hmvp_forw, HELM3D_FORW)
+HELM3D_DO_HMVP(helm3d_do_Hmvp_adj, HELM3D_ADJ)
+HELM3D_DO_HMVP(helm3d_do_Hmvp_forw_deriv, HELM3D_FORW | HELM3D_DERIV)
+HELM3D_DO_HMVP(helm3d_do_Hmvp_adj_deriv, HELM3D_ADJ | HELM3D_DERIV)
+HELM3D_DO_HMVP(helm3d_do_Hmvp_forw_wnc, HELM3D_FORW | HELM3D_WNC)
+HELM3D_DO_HMVP(helm3d_do_Hmvp_adj_wnc, HELM3D_ADJ | HELM3D_WNC)
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// Krylov vector kernels: launch interface (internal to the library).
+#pragma once
+
+#include "helm3d_common.hpp"
+
+namespace helm3d {
+
+constexpr int kMaxBasis = 32;      // largest restart length (basis slots per cycle)
+constexpr int kMaxPartials = 2048; // blocks of the streaming reductions
+
+// Up to kMaxBasis complex coefficients passed by value to a kernel.
+struct CoefPack {
+    double v[2 * kMaxBasis];
+};
+
+int krylov_blocks(long n);
+
+template <typename R>
+hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n, double* part, int nblk,
+                       hipStream_t s);
+// a_c = scl.v[c] * (d[2c] + i d[2c+1]); d is device memory.
+template <typename R>
+hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
+                         long n, double* part, int nblk, hipStream_t s);
+template <typename R>
+hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, cpx<R>* x, long n, int nblk,
+                          hipStream_t s);
+hipError_t launch_interleave(const double* re, const double* im, cpx<double>* z, long n, hipStream_t s);
+hipError_t launch_deinterleave(const cpx<double>* z, double* re, double* im, long n, hipStream_t s);
+hipError_t launch_cast_real(const double* a, float* b, long n, hipStream_t s);
+hipError_t launch_reduce(const double* part, int nblk, int stride, int nq, double* out, hipStream_t s);
+template <typename RX, typename RY>
+hipError_t launch_axpby(double ar, double ai, const cpx<RX>* x, double br, double bi, cpx<RY>* y, long n,
+                        bool use_y, hipStream_t s);
+template <typename R>
+hipError_t launch_norm2(const cpx<R>* x, long n, double* part, int nblk, hipStream_t s);
+
+}  // namespace helm3d

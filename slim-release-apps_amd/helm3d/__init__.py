@@ -1,0 +1,10 @@
+"""MI355X-native 3D frequency-domain Helmholtz hot path of SLIM-release-apps.
+
+The product is ``libhelm3d.so`` (HIP kernels for gfx950 behind the C ABI of
+``include/helm3d.h``); this package is its Python/PyTorch shim and the
+host-side mirror of the reference's MATLAB interface for this path.
+"""
+from ._lib import Helm3dError, lib, EXPORTED  # noqa: F401
+from .core import Helm3d, SolveInfo, helm3d_mvp_host, npml_flat  # noqa: F401
+
+__all__ = ["Helm3d", "SolveInfo", "Helm3dError", "helm3d_mvp_host", "npml_flat", "lib", "EXPORTED"]

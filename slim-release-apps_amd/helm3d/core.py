@@ -1,0 +1,263 @@
+"""Device-resident Helmholtz operator over the C ABI (torch tensors as buffers).
+
+``Helm3d`` owns one ``helm3d_ctx``: the model (wn on the PML grid, PML
+tables), the Krylov workspace, and the HIP stream it runs on (torch's current
+stream at call time).  Vectors are interleaved complex torch tensors on the
+context's device, x fastest (flattened in MATLAB column-major order).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def _as_int3(v) -> tuple:
+    v = tuple(int(a) for a in np.asarray(v).reshape(-1))
+    if len(v) != 3:
+        raise ValueError("expected 3 values")
+    return v
+
+
+def npml_flat(npml) -> tuple:
+    """Accept the MATLAB 2x3 [lo;hi] matrix, a 6-vector [xlo,xhi,ylo,yhi,zlo,zhi]
+    or a scalar (same count on every face)."""
+    a = np.asarray(npml)
+    if a.size == 1:
+        return (int(a),) * 6
+    if a.shape == (2, 3):
+        return tuple(int(v) for v in a.T.reshape(-1))
+    if a.size == 6:
+        return tuple(int(v) for v in a.reshape(-1))
+    raise ValueError("npml must be a scalar, a 2x3 [lo;hi] matrix or 6 values")
+
+
+@dataclass
+class SolveInfo:
+    """Outcome of a solve (FGMRES.m outputs plus counters)."""
+    iters: int
+    cycles: int
+    relres: float
+    quasi_relres: float
+    converged: bool
+    inner_iters: int = 0
+    hst: Optional[np.ndarray] = field(default=None, repr=False)
+
+
+class Helm3d:
+    """H(wn) on one device; forward / adjoint / derivative products and solves."""
+
+    def __init__(self, n, h, npml, wn: torch.Tensor, device=None):
+        if device is None:
+            device = wn.device
+        device = torch.device(device)
+        if device.type != "cuda":
+            raise RuntimeError("Helm3d runs on a HIP device; got " + str(device))
+        self.device = device
+        self._ctx = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            check(lib().helm3d_create(device.index if device.index is not None else -1,
+                                      ctypes.c_void_p(_stream_ptr()), ctypes.byref(self._ctx)))
+        self.set_model(n, h, npml, wn)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            lib().helm3d_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bind(self):
+        check(lib().helm3d_set_stream(self._ctx, ctypes.c_void_p(_stream_ptr())))
+
+    # -- model --------------------------------------------------------------
+    def set_model(self, n, h, npml, wn: torch.Tensor):
+        self.n = _as_int3(n)
+        self.h = tuple(float(a) for a in np.asarray(h, dtype=np.float64).reshape(3))
+        self.npml = npml_flat(npml)
+        self.N = int(np.prod(self.n))
+        wn = wn.reshape(-1)
+        if wn.numel() != self.N:
+            raise ValueError("wn must have prod(n) elements")
+        self.wn_complex = wn.is_complex()
+        wn = wn.to(device=self.device, dtype=torch.complex128 if self.wn_complex else torch.float64).contiguous()
+        n3 = (ctypes.c_int * 3)(*self.n)
+        h3 = (ctypes.c_double * 3)(*self.h)
+        p6 = (ctypes.c_int * 6)(*self.npml)
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_set_model(self._ctx, n3, h3, p6, ctypes.c_void_p(_dptr(wn)),
+                                         int(self.wn_complex)))
+        self.wn = wn
+
+    @property
+    def n_phys(self):
+        p = self.npml
+        return tuple(self.n[a] - p[2 * a] - p[2 * a + 1] for a in range(3))
+
+    # -- products -----------------------------------------------------------
+    def _vec(self, x: torch.Tensor, dtype) -> torch.Tensor:
+        if x.device != self.device:
+            raise ValueError(f"vector on {x.device}, operator on {self.device}")
+        x = x.reshape(-1)
+        if x.numel() != self.N:
+            raise ValueError("vector must have prod(n) elements")
+        return x.to(dtype).contiguous()
+
+    def apply(self, x: torch.Tensor, adjoint: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """y = H x (or H^H x).  complex128 input runs the FP64 kernel, complex64
+        the FP32 kernel."""
+        prec = _lib.HELM3D_C64 if x.dtype == torch.complex64 else _lib.HELM3D_C128
+        dt = torch.complex64 if prec == _lib.HELM3D_C64 else torch.complex128
+        x = self._vec(x, dt)
+        y = out if out is not None else torch.empty_like(x)
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_apply(self._ctx, _lib.HELM3D_ADJ if adjoint else _lib.HELM3D_FORW, prec,
+                                     ctypes.c_void_p(_dptr(x)), ctypes.c_void_p(_dptr(y))))
+        return y
+
+    def apply_deriv(self, dwn: torch.Tensor, x: torch.Tensor, adjoint: bool = False) -> torch.Tensor:
+        """(dH . dwn) x or its adjoint (the DERIV stencil with real dwn)."""
+        prec = _lib.HELM3D_C64 if x.dtype == torch.complex64 else _lib.HELM3D_C128
+        dt = torch.complex64 if prec == _lib.HELM3D_C64 else torch.complex128
+        rt = torch.float32 if prec == _lib.HELM3D_C64 else torch.float64
+        x = self._vec(x, dt)
+        dwn = dwn.reshape(-1).to(device=self.device, dtype=rt).contiguous()
+        if dwn.numel() != self.N:
+            raise ValueError("dwn must have prod(n) elements")
+        y = torch.empty_like(x)
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_apply_deriv(self._ctx, int(adjoint), prec, ctypes.c_void_p(_dptr(dwn)),
+                                           ctypes.c_void_p(_dptr(x)), ctypes.c_void_p(_dptr(y))))
+        return y
+
+    # -- solves -------------------------------------------------------------
+    @staticmethod
+    def _opts(tol, maxinnerit, maxit, use_x0, inner_tol=1e-2, inner_m=20, inner_maxit=200):
+        o = _lib.SolveOpts()
+        lib().helm3d_solve_opts_default(ctypes.byref(o))
+        o.tol = float(tol)
+        o.maxinnerit = int(maxinnerit)
+        o.maxit = int(maxit)
+        o.use_x0 = int(use_x0)
+        o.inner_tol = float(inner_tol)
+        o.inner_m = int(inner_m)
+        o.inner_maxit = int(inner_maxit)
+        return o
+
+    def solve(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxinnerit: int = 20,
+              maxit: int = 10000, x0: Optional[torch.Tensor] = None, return_hst: bool = False,
+              raise_on_notconv: bool = False):
+        """FGMRES.m restarted GMRES (identity preconditioner) in complex128.
+        mode 1: H x = b; otherwise H^H x = b (linearsolve.m:44-48)."""
+        b = self._vec(b, torch.complex128)
+        x = torch.zeros_like(b) if x0 is None else self._vec(x0, torch.complex128).clone()
+        o = self._opts(tol, maxinnerit, maxit, x0 is not None)
+        info = _lib.SolveInfo()
+        cap = 0
+        hst = None
+        nh = ctypes.c_longlong(0)
+        if return_hst:
+            cap = int(min(maxit * maxinnerit + 1, 1 << 22))
+            hst = np.zeros(cap)
+        with torch.cuda.device(self.device):
+            self._bind()
+            code = lib().helm3d_solve(self._ctx, int(mode), ctypes.c_void_p(_dptr(b)), ctypes.c_void_p(_dptr(x)),
+                                      ctypes.byref(o), ctypes.byref(info),
+                                      hst.ctypes.data_as(_lib._dp) if hst is not None else None, cap,
+                                      ctypes.byref(nh))
+        check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
+        si = SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
+                       bool(info.converged), hst=hst[:nh.value].copy() if hst is not None else None)
+        return x, si
+
+    def solve_mixed(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxinnerit: int = 20,
+                    maxit: int = 10000, inner_tol: float = 1e-2, inner_m: int = 20, inner_maxit: int = 200,
+                    x0: Optional[torch.Tensor] = None, raise_on_notconv: bool = False):
+        """FP64 flexible GMRES with a complex64 GMRES preconditioner (mixed precision)."""
+        b = self._vec(b, torch.complex128)
+        x = torch.zeros_like(b) if x0 is None else self._vec(x0, torch.complex128).clone()
+        o = self._opts(tol, maxinnerit, maxit, x0 is not None, inner_tol, inner_m, inner_maxit)
+        info = _lib.SolveInfo()
+        with torch.cuda.device(self.device):
+            self._bind()
+            code = lib().helm3d_solve_mixed(self._ctx, int(mode), ctypes.c_void_p(_dptr(b)),
+                                            ctypes.c_void_p(_dptr(x)), ctypes.byref(o), ctypes.byref(info))
+        check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
+        return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
+                            bool(info.converged), inner_iters=int(info.inner_iters))
+
+    # -- gradient -----------------------------------------------------------
+    def grad_accum(self, U: torch.Tensor, V: torch.Tensor, g: torch.Tensor, scale: float = 1.0,
+                   dwn: Optional[torch.Tensor] = None):
+        """g += scale * comp2phys(Re(conj(U) .* (dH^H V))) (PDEfunc.m:394)."""
+        U = self._vec(U, torch.complex128)
+        V = self._vec(V, torch.complex128)
+        if g.dtype != torch.float64 or not g.is_contiguous() or g.numel() != int(np.prod(self.n_phys)):
+            raise ValueError("g must be a contiguous float64 tensor on the physical grid")
+        d = None
+        if dwn is not None:
+            d = dwn.reshape(-1).to(device=self.device, dtype=torch.float64).contiguous()
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_fwi_grad_accum(self._ctx, ctypes.c_void_p(_dptr(d)) if d is not None else None,
+                                              ctypes.c_void_p(_dptr(U)), ctypes.c_void_p(_dptr(V)),
+                                              float(scale), ctypes.c_void_p(_dptr(g))))
+        return g
+
+
+# ---------------------------------------------------------------------------
+# Drop-in matvec at the MEX boundary (host numpy arrays)
+# ---------------------------------------------------------------------------
+
+_VARIANT_NAMES = {
+    "forw": _lib.HELM3D_FORW, "adj": _lib.HELM3D_ADJ,
+    "forw_deriv": _lib.HELM3D_FORW | _lib.HELM3D_DERIV, "adj_deriv": _lib.HELM3D_ADJ | _lib.HELM3D_DERIV,
+    "forw_wnc": _lib.HELM3D_FORW | _lib.HELM3D_WNC, "adj_wnc": _lib.HELM3D_ADJ | _lib.HELM3D_WNC,
+}
+
+
+def helm3d_mvp_host(variant, wn, h, n, npml, x, nthreads: int = 1) -> np.ndarray:
+    """y = Helm3dmvp_<variant>_mex(wn, h, n, npml, x, nthreads) on host arrays
+    (split-complex boundary of Helm3dmvp_mex.c:58-107), computed on the GPU."""
+    v = _VARIANT_NAMES[variant] if isinstance(variant, str) else int(variant)
+    wn = np.asarray(wn).reshape(-1)
+    n_arr = np.ascontiguousarray(np.asarray(n, dtype=np.float64).reshape(-1))
+    x = np.asarray(x)
+    x_rows = x.shape[0] if x.ndim >= 1 else 1
+    x = x.reshape(x_rows, -1)[:, 0] if x.ndim > 1 else x.reshape(-1)
+    N = x.size
+    wnr = np.ascontiguousarray(wn.real, dtype=np.float64)
+    wni = np.ascontiguousarray(wn.imag, dtype=np.float64) if np.iscomplexobj(wn) else None
+    xr = np.ascontiguousarray(x.real, dtype=np.float64)
+    xi = np.ascontiguousarray(x.imag, dtype=np.float64) if np.iscomplexobj(x) else None
+    hh = np.ascontiguousarray(np.asarray(h, dtype=np.float64).reshape(3))
+    pp = np.ascontiguousarray(np.asarray(npml_flat(npml), dtype=np.float64))
+    yr = np.zeros(N)
+    yi = np.zeros(N)
+    P = lambda a: a.ctypes.data_as(_lib._dp) if a is not None else None  # noqa: E731
+    code = lib().helm3d_mvp(v, P(wnr), P(wni), P(hh), P(n_arr), n_arr.size, P(pp), P(yr), P(yi), P(xr), P(xi),
+                            x_rows, int(nthreads))
+    check(code)
+    return yr + 1j * yi

@@ -1,0 +1,116 @@
+"""Synthetic velocity models and sources of the benchmark configurations.
+
+Inputs only (generated on the host with fixed seeds); the hot path never
+depends on how they were made.  Arrays are (nx, ny, nz) in physical-grid
+order; flattening for the operator uses MATLAB column-major order (x fastest),
+``vec(a) = a.reshape(-1, order='F')``.
+
+  C1  64^3 + 10-pt PML, h 10 m, 5 Hz: Gaussian-smoothed (sigma 8) random field
+      scaled to 1500-4500 m/s; unit impulse at (32,32,5).
+  C2  200x200x100 + 20-pt PML, h 10 m, 6 Hz: 1500->4000 m/s linear in depth
+      times (1 + 0.1 s), s a sigma-8 smoothed field scaled to [-1,1];
+      impulse at (100,100,5).
+  C3  C2 model, 8 impulses at z=5, y=100, x=round(linspace(20,179,8)).
+  C5  256x256x128 + 20-pt PML, h 12.5 m: smooth background (C1 recipe with
+      sigma 20) plus 12 Gaussian blobs (+-300 m/s, radius 10-25 pts); start
+      model = true model smoothed with sigma 20.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Tuple
+
+import numpy as np
+from scipy.ndimage import gaussian_filter
+
+SEED_BASE = 20261202
+
+
+def vec(a: np.ndarray) -> np.ndarray:
+    """MATLAB vec(): column-major flatten (x fastest)."""
+    return np.asarray(a).reshape(-1, order="F")
+
+
+def unvec(v: np.ndarray, n) -> np.ndarray:
+    return np.asarray(v).reshape(tuple(n), order="F")
+
+
+@dataclass
+class Config:
+    name: str
+    n_phys: Tuple[int, int, int]
+    npml: int
+    h: float
+    freq: float
+    v: np.ndarray                      # physical velocity (m/s), (nx,ny,nz)
+    sources: List[Tuple[int, int, int]]  # physical 0-based indices
+
+    @property
+    def nt(self):
+        return tuple(a + 2 * self.npml for a in self.n_phys)
+
+    @property
+    def npml6(self):
+        return (self.npml,) * 6
+
+
+def _scaled_smooth(shape, sigma, rng, lo, hi):
+    s = gaussian_filter(rng.standard_normal(shape), sigma=sigma, mode="reflect")
+    return lo + (hi - lo) * (s - s.min()) / (s.max() - s.min())
+
+
+def config_c1(seed: int = SEED_BASE + 1) -> Config:
+    rng = np.random.default_rng(seed)
+    n = (64, 64, 64)
+    v = _scaled_smooth(n, 8, rng, 1500.0, 4500.0)
+    return Config("C1", n, 10, 10.0, 5.0, v, [(32, 32, 5)])
+
+
+def _c2_velocity(seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    n = (200, 200, 100)
+    z = np.arange(n[2], dtype=np.float64)
+    vz = 1500.0 + 2500.0 * z / (n[2] - 1)
+    s = _scaled_smooth(n, 8, rng, -1.0, 1.0)
+    return vz[None, None, :] * (1.0 + 0.1 * s)
+
+
+def config_c2(seed: int = SEED_BASE + 2) -> Config:
+    return Config("C2", (200, 200, 100), 20, 10.0, 6.0, _c2_velocity(seed), [(100, 100, 5)])
+
+
+def config_c3(seed: int = SEED_BASE + 2) -> Config:
+    xs = np.round(np.linspace(20, 179, 8)).astype(int)
+    return Config("C3", (200, 200, 100), 20, 10.0, 6.0, _c2_velocity(seed), [(int(x), 100, 5) for x in xs])
+
+
+def config_small(n=(32, 32, 24), npml=8, h=10.0, freq=5.0, seed: int = SEED_BASE + 9) -> Config:
+    """Reduced C1-style case for quick parity solves."""
+    rng = np.random.default_rng(seed)
+    v = _scaled_smooth(n, 4, rng, 1500.0, 4500.0)
+    return Config("small", tuple(n), npml, h, freq, v, [(n[0] // 2, n[1] // 2, 3)])
+
+
+def extend(v: np.ndarray, npml: int) -> np.ndarray:
+    """Edge-replicating extension into the PML (opExtension mode 1)."""
+    return np.pad(v, npml, mode="edge")
+
+
+def wavenumber(v_pml: np.ndarray, freq: float, unit: str = "m/s") -> np.ndarray:
+    """param2wavenum.m:20-49 (wn only)."""
+    c = (2 * np.pi * freq) ** 2
+    if unit == "m/s":
+        return c * v_pml ** (-2)
+    if unit == "s2/m2":
+        return c * v_pml
+    if unit == "s2/km2":
+        return 1e-6 * c * v_pml
+    raise ValueError(unit)
+
+
+def point_source(cfg: Config, src) -> np.ndarray:
+    """Unit impulse on the PML grid at physical index src (vec order)."""
+    b = np.zeros(cfg.nt, dtype=np.complex128)
+    p = cfg.npml
+    b[src[0] + p, src[1] + p, src[2] + p] = 1.0
+    return vec(b)

@@ -1,0 +1,112 @@
+"""GPU parity of the 27-point stencil against the CPU oracle (all six MEX variants).
+
+Bar: <= 1e-12 relative (||y_gpu - y_oracle|| / ||y_oracle||) for complex128,
+<= 2e-6 for the complex64 kernel; adjoint identities to ~1e-14.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (n incl. PML, npml 2x3 [lo;hi], h): shapes cover several x tiles (64 wide),
+# partial y tiles (4 rows), several z chunks, asymmetric PML and a free surface.
+CASES = [
+    ((12, 10, 8), [[3, 2, 2], [3, 2, 2]], (10.0, 10.0, 10.0)),
+    ((9, 11, 7), [[2, 3, 1], [4, 2, 3]], (10.0, 12.0, 9.0)),
+    ((70, 9, 13), [[5, 2, 3], [5, 2, 3]], (12.5, 12.5, 12.5)),
+    ((130, 7, 40), [[10, 2, 6], [10, 2, 6]], (10.0, 10.0, 10.0)),
+    ((33, 35, 31), [[0, 4, 4], [4, 4, 4]], (10.0, 10.0, 10.0)),
+]
+VARIANTS = ("forw", "adj", "forw_deriv", "adj_deriv", "forw_wnc", "adj_wnc")
+
+
+def _inputs(n, seed, wnc):
+    rng = np.random.default_rng(seed)
+    N = int(np.prod(n))
+    wn = rng.uniform(2e-4, 2e-3, N)
+    if wnc:
+        wn = wn + 1j * rng.uniform(-2e-4, 2e-4, N)
+    x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    return wn, x
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_dropin_matches_oracle(case, variant):
+    from helm3d import helm3d_mvp_host
+    n, npml, h = CASES[case]
+    wn, x = _inputs(n, 100 + case, variant.endswith("wnc"))
+    print(f"dropin case={case} n={n} variant={variant}", flush=True)
+    y = helm3d_mvp_host(variant, wn, h, n, np.array(npml), x)
+    yo = O.hmvp(variant, wn, h, n, np.array(npml), x)
+    assert relerr(y, yo) <= 1e-12
+    if O.ref_available():
+        yr = O.hmvp(variant, wn, h, n, np.array(npml), x, use_ref=True)
+        assert relerr(y, yr) <= 1e-12
+
+
+def test_dropin_real_x_and_errors():
+    from helm3d import helm3d_mvp_host, Helm3dError
+    n, npml, h = CASES[0]
+    wn, x = _inputs(n, 7, False)
+    y = helm3d_mvp_host("forw", wn, h, n, np.array(npml), x.real)
+    yo = O.hmvp("forw", wn, h, n, np.array(npml), x.real.astype(complex))
+    assert relerr(y, yo) <= 1e-12
+    with pytest.raises(Helm3dError) as e:
+        helm3d_mvp_host("forw", wn, h, (12, 10), np.array(npml), x)
+    assert e.value.identifier.endswith("Helm3dmvp_mex:Nsize")
+    with pytest.raises(Helm3dError) as e:
+        helm3d_mvp_host("forw", wn, h, n, np.array(npml), x[:-1])
+    assert e.value.identifier.endswith("Helm3dkaczswp_mex:Xsize")
+
+
+@pytest.mark.parametrize("case", [1, 3, 4])
+@pytest.mark.parametrize("wnc", [False, True])
+def test_device_apply_and_adjoint(case, wnc):
+    import torch
+    from helm3d import Helm3d
+    n, npml, h = CASES[case]
+    wn, x = _inputs(n, 300 + case, wnc)
+    dev = torch.device("cuda:0")
+    print(f"device case={case} n={n} wnc={wnc}", flush=True)
+    H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).to(dev))
+    xt = torch.from_numpy(x).to(dev)
+    yf = H.apply(xt).cpu().numpy()
+    ya = H.apply(xt, adjoint=True).cpu().numpy()
+    vf = "forw_wnc" if wnc else "forw"
+    va = "adj_wnc" if wnc else "adj"
+    assert relerr(yf, O.hmvp(vf, wn, h, n, np.array(npml), x)) <= 1e-12
+    assert relerr(ya, O.hmvp(va, wn, h, n, np.array(npml), x)) <= 1e-12
+    # <H x, z> = <x, H^H z>
+    rng = np.random.default_rng(9)
+    z = rng.standard_normal(x.size) + 1j * rng.standard_normal(x.size)
+    Haz = H.apply(torch.from_numpy(z).to(dev), adjoint=True).cpu().numpy()
+    lhs = np.vdot(z, yf)
+    rhs = np.vdot(Haz, x)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    # complex64 kernel
+    y32 = H.apply(xt.to(torch.complex64)).cpu().numpy()
+    assert relerr(y32, yf) <= 2e-6
+
+
+def test_device_deriv():
+    import torch
+    from helm3d import Helm3d
+    n, npml, h = CASES[2]
+    wn, x = _inputs(n, 11, False)
+    rng = np.random.default_rng(12)
+    dwn = rng.uniform(-1e-3, 1e-3, x.size)
+    dev = torch.device("cuda:0")
+    H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).to(dev))
+    xt = torch.from_numpy(x).to(dev)
+    dt = torch.from_numpy(dwn).to(dev)
+    yf = H.apply_deriv(dt, xt).cpu().numpy()
+    ya = H.apply_deriv(dt, xt, adjoint=True).cpu().numpy()
+    assert relerr(yf, O.hmvp("forw_deriv", dwn, h, n, np.array(npml), x)) <= 1e-12
+    assert relerr(ya, O.hmvp("adj_deriv", dwn, h, n, np.array(npml), x)) <= 1e-12
