@@ -130,6 +130,13 @@ int helm3d_apply(helm3d_ctx *ctx, int variant, int precision, const void *x_dev,
 int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *dwn_dev,
                        const void *x_dev, void *y_dev);
 
+/* Stencil launch shape: z planes per block (0: automatic), column-tile rows
+ * (4 or 8: 256 or 512 threads) and planes of loads in flight (1 or 2).  The
+ * result is identical for every shape; only speed changes.  Chosen per grid by
+ * the autotuner of the Python package (helm3d.tune). */
+int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch);
+int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *blocks);
+
 /* ---- solves ------------------------------------------------------------- */
 
 /* Options of LinSolveOpts.m:55-62 / FGMRES.m:42-45 plus the mixed-precision
@@ -182,6 +189,19 @@ int helm3d_solve_mixed(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev
  * (n - npml_lo - npml_hi per axis). */
 int helm3d_fwi_grad_accum(helm3d_ctx *ctx, const double *dwn_dev, const void *U_dev,
                           const void *V_dev, double scale, double *g_dev);
+
+/* ---- in-library kernel timing ------------------------------------------ *
+ * When enabled, every kernel the context launches is bracketed by HIP events
+ * on its stream and the elapsed times are summed per kernel class.  Enabling
+ * (or disabling) resets the totals. */
+#define HELM3D_PROF_STENCIL 0  /* the 27-point stencil (matvec, residual)   */
+#define HELM3D_PROF_DOTS    1  /* Gram-Schmidt dot products                 */
+#define HELM3D_PROF_UPDATE  2  /* Gram-Schmidt update + norm                */
+#define HELM3D_PROF_COMBINE 3  /* solution update x += V y                  */
+#define HELM3D_PROF_REDUCE  4  /* second stage of the reductions            */
+#define HELM3D_PROF_KINDS   5
+int helm3d_profile(helm3d_ctx *ctx, int enable);
+int helm3d_profile_read(helm3d_ctx *ctx, int kind, double *total_ms, long long *launches);
 
 /* Library version string. */
 const char *helm3d_version(void);

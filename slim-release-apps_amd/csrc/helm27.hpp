@@ -26,15 +26,22 @@ struct StencilLaunch {
     double* gg;
     double gscale;
     int p0[3], pn[3];
+    // write-only sink of kSinkSlots values for steps/lanes with no output
+    cpx<R>* sink;
     int tiles_x, tiles_y, kchunk;
 };
 
+// Launch shape: 64 x ty column tiles, z chunks of kchunk planes, pf planes of
+// loads in flight.
+constexpr int kSinkSlots = 65536;
+
 struct StencilGrid {
-    int tiles_x, tiles_y, kchunk, nchunks, blocks;
+    int tiles_x, tiles_y, kchunk, nchunks, blocks, ty, pf;
 };
 
+// kchunk <= 0 picks a chunk giving ~2048 blocks; ty in {4, 8}; pf in {1, 2}.
 template <typename R>
-StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk);
+StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty = 4, int pf = 2);
 
 template <typename R>
 hipError_t launch_stencil(int variant, StencilLaunch<R> L, const StencilGrid& g, hipStream_t s);

@@ -142,6 +142,8 @@ struct helm3d_ctx {
 
     double* part = nullptr;
     size_t part_cap = 0;
+    void* sink = nullptr;  // stencil sink (kSinkSlots complex doubles)
+    size_t sink_cap = 0;
     double* small = nullptr;  // device: 2*kMaxBasis dots + norm
     size_t small_cap = 0;
     double* small_host = nullptr;  // pinned mirror
@@ -149,13 +151,22 @@ struct helm3d_ctx {
     Vecs<double> V64, Z64, T64;
     Vecs<float> V32, T32a, T32b;
 
+    // optional per-kernel-class timing with HIP events on `stream`
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_pending;  // (kind, index of start event)
+    double prof_ms[HELM3D_PROF_KINDS] = {0};
+    long long prof_n[HELM3D_PROF_KINDS] = {0};
+
     ~helm3d_ctx()
     {
         (void)hipSetDevice(device);
-        void* bufs[] = {tab64, tab32, wn64, wn32, part, small, V64.p, Z64.p, T64.p, V32.p, T32a.p, T32b.p};
+        void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, V32.p, T32a.p, T32b.p};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (small_host) (void)hipHostFree(small_host);
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     }
 };
 
@@ -167,6 +178,59 @@ int bind(helm3d_ctx* c)
     HIPCHK(hipSetDevice(c->device));
     return HELM3D_OK;
 }
+
+// Fold completed event pairs into the per-kind totals (stream must be idle).
+int prof_fold(helm3d_ctx* c)
+{
+    for (const auto& pr : c->ev_pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[pr.second], c->ev_pool[pr.second + 1]));
+        c->prof_ms[pr.first] += ms;
+        c->prof_n[pr.first] += 1;
+    }
+    c->ev_pending.clear();
+    c->ev_used = 0;
+    return HELM3D_OK;
+}
+
+// Record the start of a timed launch of class `kind`; returns the slot.
+int prof_begin(helm3d_ctx* c, int kind, long* slot)
+{
+    *slot = -1;
+    if (!c->prof) return HELM3D_OK;
+    if (c->ev_used + 2 > c->ev_pool.size()) {
+        if (c->ev_pool.size() >= 8192) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            CHK(prof_fold(c));
+        } else {
+            for (int q = 0; q < 256; ++q) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                c->ev_pool.push_back(e);
+            }
+        }
+    }
+    *slot = long(c->ev_used);
+    c->ev_pending.push_back({kind, c->ev_used});
+    c->ev_used += 2;
+    HIPCHK(hipEventRecord(c->ev_pool[*slot], c->stream));
+    return HELM3D_OK;
+}
+int prof_end(helm3d_ctx* c, long slot)
+{
+    if (slot < 0) return HELM3D_OK;
+    HIPCHK(hipEventRecord(c->ev_pool[slot + 1], c->stream));
+    return HELM3D_OK;
+}
+
+// Wrap a launch expression returning hipError_t with profiling of `kind`.
+#define TIMED(kind, expr)                   \
+    do {                                    \
+        long slot_;                         \
+        CHK(prof_begin(c, (kind), &slot_)); \
+        HIPCHK(expr);                       \
+        CHK(prof_end(c, slot_));            \
+    } while (0)
 
 template <typename R>
 OpDesc<R>& op_of(helm3d_ctx* c);
@@ -191,6 +255,7 @@ int ensure_work(helm3d_ctx* c)
     const size_t nblk = std::max<size_t>(c->grid.blocks, kMaxPartials);
     CHK(ensure_t(&c->part, &c->part_cap, nblk * 2 * kMaxBasis));
     CHK(ensure_t(&c->small, &c->small_cap, 2 * kMaxBasis + 8));
+    CHK(ensure(&c->sink, &c->sink_cap, size_t(kSinkSlots) * sizeof(cpx<double>)));
     if (!c->small_host) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->small_host), (2 * kMaxBasis + 8) * sizeof(double)));
     return HELM3D_OK;
 }
@@ -214,7 +279,8 @@ int stencil(helm3d_ctx* c, int variant, const OpDesc<R>& op, const cpx<R>* x, cp
     L.alpha = alpha;
     L.beta = beta;
     L.norm_part = norm_part;
-    HIPCHK(launch_stencil<R>(variant, L, c->grid, c->stream));
+    L.sink = static_cast<cpx<R>*>(c->sink);
+    TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, c->stream));
     return HELM3D_OK;
 }
 
@@ -223,17 +289,17 @@ int model_variant(const helm3d_ctx* c, int mode_adj)
     return (mode_adj ? VAR_ADJ : VAR_FORW) | (c->wnc ? VAR_WNC : 0);
 }
 
-// Reduce `nq` partial columns into c->small[off..off+nq) and (optionally)
-// fetch small[0..nfetch) to the pinned host mirror.
-int reduce_to_small(helm3d_ctx* c, int nblk, int stride, int nq, int off)
+// Reduce `nq` partial columns (part[q*nblk + b]) into c->small[off..off+nq).
+int reduce_to_small(helm3d_ctx* c, int nblk, int nq, int off)
 {
-    HIPCHK(launch_reduce(c->part, nblk, stride, nq, c->small + off, c->stream));
+    TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, nq, c->small + off, c->stream));
     return HELM3D_OK;
 }
 int fetch_small(helm3d_ctx* c, int count)
 {
     HIPCHK(hipMemcpyAsync(c->small_host, c->small, count * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->prof) CHK(prof_fold(c));
     return HELM3D_OK;
 }
 
@@ -278,12 +344,12 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62)
     if (use_x0) {
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
-        CHK(reduce_to_small(c, c->grid.blocks, 1, 1, 0));
+        CHK(reduce_to_small(c, c->grid.blocks, 1, 0));
     } else {
         HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, slot(0), N, false, c->stream)));
         HIPCHK((launch_axpby<R, R>(0.0, 0.0, static_cast<const cpx<R>*>(nullptr), 0.0, 0.0, x, N, false, c->stream)));
         HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
-        CHK(reduce_to_small(c, nblk, 1, 1, 0));
+        CHK(reduce_to_small(c, nblk, 1, 0));
     }
     CHK(fetch_small(c, 1));
     const double normr0 = std::sqrt(c->small_host[0]);
@@ -317,20 +383,20 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
                 CHK(stencil<R>(c, variant, op, slot(k - 1), slot(k), R(sig[k - 1]), nullptr, R(0), nullptr));
             }
             // classical Gram-Schmidt, FGMRES.m:100-103
-            HIPCHK(launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
-            CHK(reduce_to_small(c, nblk, 2 * kMaxBasis, 2 * k, 0));
+            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
+            CHK(reduce_to_small(c, nblk, 2 * k, 0));
             CoefPack scl{};
             for (int q = 0; q < k; ++q) scl.v[q] = sig[q] * sig[q];
-            HIPCHK(launch_update<R>(V, ld, k, c->small, scl, slot(k), N, c->part, nblk, c->stream));
-            CHK(reduce_to_small(c, nblk, 1, 1, 2 * kMaxBasis));
-            // fetch d (2k values) and ||w||^2
-            HIPCHK(hipMemcpyAsync(c->small_host, c->small, 2 * k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipMemcpyAsync(c->small_host + 2 * kMaxBasis, c->small + 2 * kMaxBasis, sizeof(double),
-                                  hipMemcpyDeviceToHost, c->stream));
+            TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, k, c->small, scl, slot(k), N, c->part, nblk, c->stream));
+            CHK(reduce_to_small(c, nblk, 1, 2 * k));
+            // fetch d (2k values) and ||w||^2 in one copy
+            HIPCHK(hipMemcpyAsync(c->small_host, c->small, (2 * k + 1) * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->prof) CHK(prof_fold(c));
             for (int q = 0; q < k; ++q)
                 H[q + (k - 1) * (m + 1)] = sig[q] * C(c->small_host[2 * q], c->small_host[2 * q + 1]);
-            const double hn = std::sqrt(c->small_host[2 * kMaxBasis]);
+            const double hn = std::sqrt(c->small_host[2 * k]);
             H[k + (k - 1) * (m + 1)] = hn;
             sig[k] = 1.0 / hn;
             hlast = hess_lsq(k, m + 1, H, beta, y) / normr0;
@@ -351,13 +417,14 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             a.v[2 * q + 1] = v.imag();
         }
         if (P) {
-            if constexpr (std::is_same<R, double>::value) HIPCHK(launch_combine<double>(Z, ld, k - 1, a, x, N, nblk, c->stream));
+            if constexpr (std::is_same<R, double>::value)
+                TIMED(HELM3D_PROF_COMBINE, launch_combine<double>(Z, ld, k - 1, a, x, N, nblk, c->stream));
         } else {
-            HIPCHK(launch_combine<R>(V, ld, k - 1, a, x, N, nblk, c->stream));
+            TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(V, ld, k - 1, a, x, N, nblk, c->stream));
         }
         // true residual r = b - A x into slot 0, FGMRES.m:120-124
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
-        CHK(reduce_to_small(c, c->grid.blocks, 1, 1, 0));
+        CHK(reduce_to_small(c, c->grid.blocks, 1, 0));
         CHK(fetch_small(c, 1));
         const double rn = std::sqrt(c->small_host[0]);
         res->relres = rn / normr0;
@@ -409,6 +476,30 @@ helm3d_ctx* g_dropin[64] = {nullptr};
 extern "C" {
 
 const char* helm3d_version(void) { return "helm3d-mi355x 0.1"; }
+
+int helm3d_profile(helm3d_ctx* c, int enable)
+{
+    CHK(bind(c));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    CHK(prof_fold(c));
+    for (int k = 0; k < HELM3D_PROF_KINDS; ++k) {
+        c->prof_ms[k] = 0;
+        c->prof_n[k] = 0;
+    }
+    c->prof = enable != 0;
+    return HELM3D_OK;
+}
+
+int helm3d_profile_read(helm3d_ctx* c, int kind, double* total_ms, long long* launches)
+{
+    CHK(bind(c));
+    if (kind < 0 || kind >= HELM3D_PROF_KINDS || !total_ms || !launches) return fail(HELM3D_ERR_ARG, "bad kind");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    CHK(prof_fold(c));
+    *total_ms = c->prof_ms[kind];
+    *launches = c->prof_n[kind];
+    return HELM3D_OK;
+}
 
 const char* helm3d_last_error(void) { return g_err.c_str(); }
 
@@ -476,6 +567,30 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
         for (int t = 0; t < n[a] + 2; ++t, ++pos) t64[pos] = cpx<double>{ax.hi[t].real(), ax.hi[t].imag()};
     }
     for (size_t q = 0; q < len; ++q) t32[q] = cpx<float>{float(t64[q].re), float(t64[q].im)};
+    // interior index range per axis: slots i..i+2 all exactly (1, 1)
+    int ilo[3], ihi[3];
+    for (int a = 0; a < 3; ++a) {
+        auto one = [&](size_t q) { return t64[q].re == 1.0 && t64[q].im == 0.0; };
+        auto ok = [&](int i) {
+            for (int sl = i; sl <= i + 2; ++sl)
+                if (!one(off[2 * a] + sl) || !one(off[2 * a + 1] + sl)) return false;
+            return true;
+        };
+        ilo[a] = 1;
+        ihi[a] = 0;
+        int first = -1, last = -1;
+        for (int i = 0; i < n[a]; ++i)
+            if (ok(i)) {
+                if (first < 0) first = i;
+                last = i;
+            }
+        bool contiguous = first >= 0;
+        for (int i = first; contiguous && i <= last; ++i) contiguous = ok(i);
+        if (contiguous) {
+            ilo[a] = first;
+            ihi[a] = last;
+        }
+    }
     CHK(ensure_t(&c->tab64, &c->tab64_cap, len));
     CHK(ensure_t(&c->tab32, &c->tab32_cap, len));
     HIPCHK(hipMemcpyAsync(c->tab64, t64.data(), len * sizeof(cpx<double>), hipMemcpyHostToDevice, c->stream));
@@ -520,6 +635,20 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
         op.wm2 = R(kWM2);
         op.wm3 = R(kWM3);
         op.wm4 = R(kWM4);
+        // interior coefficients: xi_x = xi_y = 2, side values 1
+        const HConst& k = c->hc;
+        op.fi[0] = R(-2 * k.c[0] - 2 * k.c[1]);
+        op.fi[1] = R(k.c[0] + 2 * k.xy);
+        op.fi[2] = R(k.c[1] + 2 * k.xy);
+        op.fi[3] = R(-2 * k.xy);
+        op.gi[0] = R(2 * k.xz + 2 * k.yz);
+        op.gi[1] = R(-k.xz + 2 * k.w3a2);
+        op.gi[2] = R(-k.yz + 2 * k.w3a2);
+        op.gi[3] = R(-2 * k.w3a2);
+        for (int a = 0; a < 3; ++a) {
+            op.int_lo[a] = ilo[a];
+            op.int_hi[a] = ihi[a];
+        }
     };
     fill(c->op64, c->tab64, c->wn64);
     fill(c->op32, c->tab32, c->wn32);
@@ -527,6 +656,26 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
     CHK(ensure_work(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_model = true;
+    return HELM3D_OK;
+}
+
+int helm3d_set_launch(helm3d_ctx* c, int kchunk, int tile_rows, int prefetch)
+{
+    CHK(check_model(c));
+    if (!(tile_rows == 4 || tile_rows == 8) || !(prefetch == 1 || prefetch == 2) || kchunk < 0)
+        return fail(HELM3D_ERR_ARG, "tile_rows in {4,8}, prefetch in {1,2}, kchunk >= 0");
+    c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kchunk, tile_rows, prefetch);
+    CHK(ensure_work(c));
+    return HELM3D_OK;
+}
+
+int helm3d_get_launch(helm3d_ctx* c, int* kchunk, int* tile_rows, int* prefetch, int* blocks)
+{
+    CHK(check_model(c));
+    if (kchunk) *kchunk = c->grid.kchunk;
+    if (tile_rows) *tile_rows = c->grid.ty;
+    if (prefetch) *prefetch = c->grid.pf;
+    if (blocks) *blocks = c->grid.blocks;
     return HELM3D_OK;
 }
 
@@ -652,6 +801,7 @@ int helm3d_fwi_grad_accum(helm3d_ctx* c, const double* dwn, const void* U, const
     L.alpha = 1.0;
     L.gu = static_cast<const cpx<double>*>(U);
     L.gg = g;
+    L.sink = static_cast<cpx<double>*>(c->sink);
     L.gscale = scale;
     for (int a = 0; a < 3; ++a) {
         L.p0[a] = c->npml[2 * a];

@@ -103,6 +103,12 @@ struct OpDesc {
     R wn_const;
     // constants
     R c[3], xy, xz, yz, w3a2, wn_coef, wm2, wm3, wm4;
+    // interior (no x/y PML) column coefficients on (centre, x sum, y sum,
+    // diagonal sum): F for the own plane, G for planes k+-1
+    R fi[4], gi[4];
+    // per axis, indices i whose PML slots i, i+1, i+2 (indices i-1..i+1) all
+    // hold xi_lo = xi_hi = 1 exactly; empty when lo > hi
+    int int_lo[3], int_hi[3];
 };
 
 // Status codes returned by every C-ABI entry point.

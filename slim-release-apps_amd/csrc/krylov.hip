@@ -6,11 +6,16 @@
 //   dots    d_c = v_c^H w, c < k            (reads w and k slots)
 //   update  w <- w - sum_c a_c v_c, ||w||^2  (reads w and k slots, writes w)
 // i.e. the classical Gram-Schmidt of FGMRES.m:100-103 in two HBM passes.
+// Each kernel is instantiated for the exact basis count k (1..31) so all k+1
+// loads of a point are issued back to back, and every thread streams two
+// points per iteration.
 //
 // Reductions are deterministic: every block reduces a fixed index range in a
-// fixed order into a partial slot, and a single block sums the partials in
-// block order; results do not depend on scheduling.  Partial sums are FP64 for
-// both storage precisions.
+// fixed order into a partial slot (column-major: part[q*nblk + block]), and a
+// second kernel with one block per column sums the partials in block order.
+// Partial sums are FP64 for both storage precisions.
+#include <type_traits>
+
 #include "helm27.hpp"
 #include "krylov.hpp"
 
@@ -26,9 +31,9 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Block-wide sum of NV values per thread; result valid in thread 0 slots.
+// Block-wide sums of NV values per thread; results valid in thread 0.
 template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double* red /* [4][NV] */)
+__device__ inline void block_sum(double (&v)[NV], double* red /* [KB/64][NV] */)
 {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -48,18 +53,27 @@ __device__ inline void block_sum(double (&v)[NV], double* red /* [4][NV] */)
     }
 }
 
-// Contiguous range of the block: [lo, hi)
+// Contiguous range [lo, hi) of this block (even length, for paired points).
 __device__ inline void block_range(long n, long& lo, long& hi)
 {
-    const long per = (n + gridDim.x - 1) / gridDim.x;
+    const long per = ((n + gridDim.x - 1) / gridDim.x + 1) & ~1L;
     lo = long(blockIdx.x) * per;
     hi = min(n, lo + per);
 }
 
-// d_c = sum conj(v_c) w for c < k (K compile-time upper bound); partials are
-// laid out with the fixed stride 2*kMaxBasis per block.
+template <typename R>
+__device__ inline void acc_dot(double& re, double& im, cpx<R> v, cpx<R> w)
+{
+    // += conj(v) * w
+    re = fma(double(v.re), double(w.re), re);
+    re = fma(double(v.im), double(w.im), re);
+    im = fma(double(v.re), double(w.im), im);
+    im = fma(-double(v.im), double(w.re), im);
+}
+
+// d_c = sum conj(v_c) w, c < K.
 template <typename R, int K>
-__global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, long ldv, int k,
+__global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                   const cpx<R>* __restrict__ w, long n,
                                                   double* __restrict__ part)
 {
@@ -69,115 +83,134 @@ __global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, 
     for (int q = 0; q < 2 * K; ++q) acc[q] = 0.0;
     long lo, hi;
     block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
-        const cpx<R> wt = w[t];
+    long t = lo + 2 * threadIdx.x;
+    for (; t + 1 < hi; t += 2 * KB) {
+        const cpx<R> w0 = w[t], w1 = w[t + 1];
+        cpx<R> v0[K], v1[K];
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            if (c < k) {
-                const cpx<R> v = V[c * ldv + t];
-                acc[2 * c] += double(v.re) * double(wt.re) + double(v.im) * double(wt.im);
-                acc[2 * c + 1] += double(v.re) * double(wt.im) - double(v.im) * double(wt.re);
-            }
+            v0[c] = V[c * ldv + t];
+            v1[c] = V[c * ldv + t + 1];
         }
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            acc_dot(acc[2 * c], acc[2 * c + 1], v0[c], w0);
+            acc_dot(acc[2 * c], acc[2 * c + 1], v1[c], w1);
+        }
+    }
+    if (t < hi) {
+        const cpx<R> w0 = w[t];
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc_dot(acc[2 * c], acc[2 * c + 1], V[c * ldv + t], w0);
     }
     block_sum<2 * K>(acc, red);
     if (threadIdx.x == 0) {
-        for (int q = 0; q < 2 * k; ++q) part[long(blockIdx.x) * 2 * kMaxBasis + q] = acc[q];
+#pragma unroll
+        for (int q = 0; q < 2 * K; ++q) part[long(q) * gridDim.x + blockIdx.x] = acc[q];
     }
 }
 
-// w <- w - sum_{c<k} a_c v_c with a_c = scl_c * d_c, d read from device memory
-// (the reduced dot products), so the host need not see d before this pass;
-// partial ||w||^2.
+// w <- w - sum_{c<K} a_c v_c with a_c = scl_c * d_c (d: reduced dots in device
+// memory, so the host need not see them before this pass); partial ||w||^2.
 template <typename R, int K>
-__global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V, long ldv, int k,
+__global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                     const double* __restrict__ d, CoefPack scl,
                                                     cpx<R>* __restrict__ w, long n, double* __restrict__ part)
 {
     __shared__ double red[KB / 64];
     double s2[1] = {0.0};
-    double a[2 * K];
+    double ar[K], ai[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-        a[2 * c] = c < k ? scl.v[c] * d[2 * c] : 0.0;
-        a[2 * c + 1] = c < k ? scl.v[c] * d[2 * c + 1] : 0.0;
+        ar[c] = scl.v[c] * d[2 * c];
+        ai[c] = scl.v[c] * d[2 * c + 1];
     }
-    long lo, hi;
-    block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
-        cpx<R> wt = w[t];
+    auto upd = [&](cpx<R> wt, const cpx<R>(&v)[K]) {
         double re = wt.re, im = wt.im;
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            if (c < k) {
-                const cpx<R> v = V[c * ldv + t];
-                const double ar = a[2 * c], ai = a[2 * c + 1];
-                re -= ar * double(v.re) - ai * double(v.im);
-                im -= ar * double(v.im) + ai * double(v.re);
-            }
+            re = fma(-ar[c], double(v[c].re), re);
+            re = fma(ai[c], double(v[c].im), re);
+            im = fma(-ar[c], double(v[c].im), im);
+            im = fma(-ai[c], double(v[c].re), im);
         }
-        wt.re = R(re);
-        wt.im = R(im);
-        w[t] = wt;
-        s2[0] += double(wt.re) * double(wt.re) + double(wt.im) * double(wt.im);
+        const cpx<R> o{R(re), R(im)};
+        s2[0] += double(o.re) * double(o.re) + double(o.im) * double(o.im);
+        return o;
+    };
+    long lo, hi;
+    block_range(n, lo, hi);
+    long t = lo + 2 * threadIdx.x;
+    for (; t + 1 < hi; t += 2 * KB) {
+        const cpx<R> w0 = w[t], w1 = w[t + 1];
+        cpx<R> v0[K], v1[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            v0[c] = V[c * ldv + t];
+            v1[c] = V[c * ldv + t + 1];
+        }
+        w[t] = upd(w0, v0);
+        w[t + 1] = upd(w1, v1);
+    }
+    if (t < hi) {
+        cpx<R> v0[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) v0[c] = V[c * ldv + t];
+        w[t] = upd(w[t], v0);
     }
     block_sum<1>(s2, red);
     if (threadIdx.x == 0) part[blockIdx.x] = s2[0];
 }
 
-// x <- x + sum_{c<k} a_c v_c (the solution update of FGMRES.m:115-119).
+// x <- x + sum_{c<K} a_c v_c (the solution update of FGMRES.m:115-119).
 template <typename R, int K>
-__global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ V, long ldv, int k,
-                                                     CoefPack a, cpx<R>* __restrict__ x, long n)
+__global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ V, long ldv, CoefPack a,
+                                                     cpx<R>* __restrict__ x, long n)
 {
     long lo, hi;
     block_range(n, lo, hi);
     for (long t = lo + threadIdx.x; t < hi; t += KB) {
-        cpx<R> xt = x[t];
+        cpx<R> v[K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) v[c] = V[c * ldv + t];
+        const cpx<R> xt = x[t];
         double re = xt.re, im = xt.im;
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            if (c < k) {
-                const cpx<R> v = V[c * ldv + t];
-                const double ar = a.v[2 * c], ai = a.v[2 * c + 1];
-                re += ar * double(v.re) - ai * double(v.im);
-                im += ar * double(v.im) + ai * double(v.re);
-            }
+            const double pr = a.v[2 * c], pi = a.v[2 * c + 1];
+            re = fma(pr, double(v[c].re), re);
+            re = fma(-pi, double(v[c].im), re);
+            im = fma(pr, double(v[c].im), im);
+            im = fma(pi, double(v[c].re), im);
         }
-        xt.re = R(re);
-        xt.im = R(im);
-        x[t] = xt;
+        x[t] = cpx<R>{R(re), R(im)};
     }
 }
 
-// Sum part[b*stride + q] over b in block order for q < nq; out[q] = sum.
-__global__ __launch_bounds__(KB) void reduce_kernel(const double* __restrict__ part, int nblk, int stride,
-                                                    int nq, double* __restrict__ out)
+// out[q] = sum_b part[q*nblk + b] in block order; one block per column q.
+__global__ __launch_bounds__(KB) void reduce_kernel(const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ out)
 {
     __shared__ double red[KB];
-    for (int q = 0; q < nq; ++q) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblk; b += KB) s += part[long(b) * stride + q];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = KB / 2; o > 0; o >>= 1) {
-            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[q] = red[0];
+    const int q = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += KB) s += part[long(q) * nblk + b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = KB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[q] = red[0];
 }
 
-// y = alpha x + beta y (complex alpha, beta); beta == 0 ignores y's contents.
+// y = alpha x + beta y (complex alpha, beta); use_y == 0 ignores y's contents.
 template <typename RX, typename RY>
 __global__ __launch_bounds__(KB) void axpby_kernel(double ar, double ai, const cpx<RX>* __restrict__ x,
                                                    double br, double bi, cpx<RY>* __restrict__ y,
                                                    long n, int use_y)
 {
-    long lo, hi;
-    block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
+    for (long t = long(blockIdx.x) * KB + threadIdx.x; t < n; t += long(gridDim.x) * KB) {
         const cpx<RX> xv = x ? x[t] : cpx<RX>{RX(0), RX(0)};
         double re = ar * double(xv.re) - ai * double(xv.im);
         double im = ar * double(xv.im) + ai * double(xv.re);
@@ -228,70 +261,66 @@ __global__ __launch_bounds__(KB) void cast_real_kernel(const double* __restrict_
     for (long t = long(blockIdx.x) * KB + threadIdx.x; t < n; t += long(gridDim.x) * KB) b[t] = float(a[t]);
 }
 
-template <typename R, int K>
-hipError_t dots_k(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n, double* part, int nblk,
-                  hipStream_t s)
+// Runtime k -> fn(integral_constant<int, k>) for k in [K, kMaxBasis).
+template <int K, typename Fn>
+hipError_t dispatch_k(int k, Fn&& fn)
 {
-    hipLaunchKernelGGL((dots_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, k, w, n, part);
-    return hipGetLastError();
-}
-template <typename R, int K>
-hipError_t update_k(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& a, cpx<R>* w, long n,
-                    double* part, int nblk, hipStream_t s)
-{
-    hipLaunchKernelGGL((update_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, k, d, a, w, n, part);
-    return hipGetLastError();
-}
-template <typename R, int K>
-hipError_t combine_k(const cpx<R>* V, long ldv, int k, const CoefPack& a, cpx<R>* x, long n, int nblk,
-                     hipStream_t s)
-{
-    hipLaunchKernelGGL((combine_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, k, a, x, n);
-    return hipGetLastError();
+    if constexpr (K >= kMaxBasis) {
+        return hipErrorInvalidValue;
+    } else {
+        if (k == K) return fn(std::integral_constant<int, K>{});
+        return dispatch_k<K + 1>(k, fn);
+    }
 }
 
 }  // namespace
 
 int krylov_blocks(long n)
 {
-    long b = (n + KB * 4 - 1) / (KB * 4);
+    long b = (n + KB * 8 - 1) / (KB * 8);
     return int(std::max(1L, std::min(b, long(kMaxPartials))));
 }
 
-// Dispatch on k to a kernel whose register accumulators cover k (bucketed).
 template <typename R>
 hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n, double* part, int nblk,
                        hipStream_t s)
 {
-    if (k <= 4) return dots_k<R, 4>(V, ldv, k, w, n, part, nblk, s);
-    if (k <= 8) return dots_k<R, 8>(V, ldv, k, w, n, part, nblk, s);
-    if (k <= 16) return dots_k<R, 16>(V, ldv, k, w, n, part, nblk, s);
-    if (k <= 24) return dots_k<R, 24>(V, ldv, k, w, n, part, nblk, s);
-    if (k <= kMaxBasis) return dots_k<R, kMaxBasis>(V, ldv, k, w, n, part, nblk, s);
-    return hipErrorInvalidValue;
+    if (k < 1) return hipErrorInvalidValue;
+    return dispatch_k<1>(k, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        hipLaunchKernelGGL((dots_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part);
+        return hipGetLastError();
+    });
 }
+
 template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
                          long n, double* part, int nblk, hipStream_t s)
 {
-    if (k <= 4) return update_k<R, 4>(V, ldv, k, d, scl, w, n, part, nblk, s);
-    if (k <= 8) return update_k<R, 8>(V, ldv, k, d, scl, w, n, part, nblk, s);
-    if (k <= 16) return update_k<R, 16>(V, ldv, k, d, scl, w, n, part, nblk, s);
-    if (k <= 24) return update_k<R, 24>(V, ldv, k, d, scl, w, n, part, nblk, s);
-    if (k <= kMaxBasis) return update_k<R, kMaxBasis>(V, ldv, k, d, scl, w, n, part, nblk, s);
-    return hipErrorInvalidValue;
+    if (k < 1) return hipErrorInvalidValue;
+    return dispatch_k<1>(k, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        hipLaunchKernelGGL((update_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, w, n, part);
+        return hipGetLastError();
+    });
 }
+
 template <typename R>
 hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, cpx<R>* x, long n, int nblk,
                           hipStream_t s)
 {
     if (k <= 0) return hipSuccess;
-    if (k <= 4) return combine_k<R, 4>(V, ldv, k, a, x, n, nblk, s);
-    if (k <= 8) return combine_k<R, 8>(V, ldv, k, a, x, n, nblk, s);
-    if (k <= 16) return combine_k<R, 16>(V, ldv, k, a, x, n, nblk, s);
-    if (k <= 24) return combine_k<R, 24>(V, ldv, k, a, x, n, nblk, s);
-    if (k <= kMaxBasis) return combine_k<R, kMaxBasis>(V, ldv, k, a, x, n, nblk, s);
-    return hipErrorInvalidValue;
+    return dispatch_k<1>(k, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        hipLaunchKernelGGL((combine_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, a, x, n);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_kernel, dim3(nq), dim3(KB), 0, s, part, nblk, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_interleave(const double* re, const double* im, cpx<double>* z, long n, hipStream_t s)
@@ -307,12 +336,6 @@ hipError_t launch_deinterleave(const cpx<double>* z, double* re, double* im, lon
 hipError_t launch_cast_real(const double* a, float* b, long n, hipStream_t s)
 {
     hipLaunchKernelGGL(cast_real_kernel, dim3(krylov_blocks(n)), dim3(KB), 0, s, a, b, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_reduce(const double* part, int nblk, int stride, int nq, double* out, hipStream_t s)
-{
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(KB), 0, s, part, nblk, stride, nq, out);
     return hipGetLastError();
 }
 

@@ -28,7 +28,8 @@ hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, c
 hipError_t launch_interleave(const double* re, const double* im, cpx<double>* z, long n, hipStream_t s);
 hipError_t launch_deinterleave(const cpx<double>* z, double* re, double* im, long n, hipStream_t s);
 hipError_t launch_cast_real(const double* a, float* b, long n, hipStream_t s);
-hipError_t launch_reduce(const double* part, int nblk, int stride, int nq, double* out, hipStream_t s);
+// out[q] = sum_b part[q*nblk + b] for q < nq (block order, deterministic)
+hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s);
 template <typename RX, typename RY>
 hipError_t launch_axpby(double ar, double ai, const cpx<RX>* x, double br, double bi, cpx<RY>* y, long n,
                         bool use_y, hipStream_t s);

@@ -85,7 +85,12 @@ _SIGS = [
     ("helm3d_solve_mixed", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
                                           ctypes.POINTER(SolveInfo)]),
     ("helm3d_fwi_grad_accum", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
+    ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
+    ("helm3d_profile", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("helm3d_profile_read", ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 ]
+PROF_KINDS = {"stencil": 0, "dots": 1, "update": 2, "combine": 3, "reduce": 4}
 for _v in ("forw", "adj", "forw_deriv", "adj_deriv", "forw_wnc", "adj_wnc"):
     _SIGS.append((f"helm3d_do_Hmvp_{_v}", ctypes.c_int, [_dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                                         ctypes.c_int]))

@@ -208,6 +208,37 @@ class Helm3d:
         return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
                             bool(info.converged), inner_iters=int(info.inner_iters))
 
+    # -- launch shape ---------------------------------------------------------
+    def set_launch(self, kchunk: int = 0, tile_rows: int = 4, prefetch: int = 1):
+        """Stencil launch shape (results are identical for every shape)."""
+        with torch.cuda.device(self.device):
+            check(lib().helm3d_set_launch(self._ctx, int(kchunk), int(tile_rows), int(prefetch)))
+
+    def get_launch(self) -> dict:
+        v = [ctypes.c_int(0) for _ in range(4)]
+        with torch.cuda.device(self.device):
+            check(lib().helm3d_get_launch(self._ctx, *[ctypes.byref(a) for a in v]))
+        return dict(kchunk=v[0].value, tile_rows=v[1].value, prefetch=v[2].value, blocks=v[3].value)
+
+    # -- in-library kernel timing -------------------------------------------
+    def profile(self, enable: bool = True):
+        """Enable/disable HIP-event timing of every kernel class (resets totals)."""
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_profile(self._ctx, int(enable)))
+
+    def profile_read(self) -> dict:
+        """{kind: (total_ms, launches)} since the last profile() call."""
+        out = {}
+        with torch.cuda.device(self.device):
+            self._bind()
+            for name, k in _lib.PROF_KINDS.items():
+                ms = ctypes.c_double(0)
+                n = ctypes.c_longlong(0)
+                check(lib().helm3d_profile_read(self._ctx, k, ctypes.byref(ms), ctypes.byref(n)))
+                out[name] = (ms.value, n.value)
+        return out
+
     # -- gradient -----------------------------------------------------------
     def grad_accum(self, U: torch.Tensor, V: torch.Tensor, g: torch.Tensor, scale: float = 1.0,
                    dwn: Optional[torch.Tensor] = None):

@@ -18,6 +18,8 @@ CASES = [
     ((70, 9, 13), [[5, 2, 3], [5, 2, 3]], (12.5, 12.5, 12.5)),
     ((130, 7, 40), [[10, 2, 6], [10, 2, 6]], (10.0, 10.0, 10.0)),
     ((33, 35, 31), [[0, 4, 4], [4, 4, 4]], (10.0, 10.0, 10.0)),
+    # whole 64-wide waves outside the x/y PML (interior path) next to PML waves
+    ((200, 14, 10), [[4, 2, 2], [5, 3, 2]], (10.0, 11.0, 12.0)),
 ]
 VARIANTS = ("forw", "adj", "forw_deriv", "adj_deriv", "forw_wnc", "adj_wnc")
 
@@ -66,7 +68,7 @@ def test_dropin_real_x_and_errors():
     assert e.value.identifier.endswith("Helm3dkaczswp_mex:Xsize")
 
 
-@pytest.mark.parametrize("case", [1, 3, 4])
+@pytest.mark.parametrize("case", [1, 3, 4, 5])
 @pytest.mark.parametrize("wnc", [False, True])
 def test_device_apply_and_adjoint(case, wnc):
     import torch
@@ -110,3 +112,39 @@ def test_device_deriv():
     ya = H.apply_deriv(dt, xt, adjoint=True).cpu().numpy()
     assert relerr(yf, O.hmvp("forw_deriv", dwn, h, n, np.array(npml), x)) <= 1e-12
     assert relerr(ya, O.hmvp("adj_deriv", dwn, h, n, np.array(npml), x)) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(0, 4, 1), (35, 4, 2), (70, 8, 1), (17, 8, 2)])
+def test_launch_shapes_identical(shape):
+    """Every launch shape gives bit-identical results (interior and PML waves)."""
+    import torch
+    from helm3d import Helm3d
+    n, npml, h = CASES[5]
+    wn, x = _inputs(n, 55, False)
+    dev = torch.device("cuda:0")
+    H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).to(dev))
+    xt = torch.from_numpy(x).to(dev)
+    y0 = H.apply(xt).clone()
+    a0 = H.apply(xt, adjoint=True).clone()
+    H.set_launch(*shape)
+    assert torch.equal(H.apply(xt), y0)
+    assert torch.equal(H.apply(xt, adjoint=True), a0)
+    assert relerr(y0.cpu().numpy(), O.hmvp("forw", wn, h, n, np.array(npml), x)) <= 1e-12
+
+
+def test_c2_full_size_matvec_parity():
+    """The benchmark operator (240x240x140, PML 20) at full size, fwd and adj."""
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_c2, extend, wavenumber, vec
+    cfg = config_c2()
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    rng = np.random.default_rng(2026)
+    x = rng.standard_normal(wn.size) + 1j * rng.standard_normal(wn.size)
+    dev = torch.device("cuda:0")
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    xt = torch.from_numpy(x).to(dev)
+    for adj, v in ((False, "forw"), (True, "adj")):
+        y = H.apply(xt, adjoint=adj).cpu().numpy()
+        yo = O.hmvp(v, wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), x)
+        assert relerr(y, yo) <= 1e-12, v
