@@ -137,6 +137,12 @@ int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *
 int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch);
 int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *blocks);
 
+/* Context options.  HELM3D_OPT_GRAPHS (default 1): replay fixed kernel
+ * sequences (the fixed-length inner solve of helm3d_solve_mixed) from captured
+ * HIP graphs; 0 launches them one by one (identical results). */
+#define HELM3D_OPT_GRAPHS 1
+int helm3d_set_option(helm3d_ctx *ctx, int option, long long value);
+
 /* ---- solves ------------------------------------------------------------- */
 
 /* Options of LinSolveOpts.m:55-62 / FGMRES.m:42-45 plus the mixed-precision
@@ -147,10 +153,16 @@ typedef struct {
     long long maxit;    /* maximum number of restart cycles (10000)                    */
     int    use_x0;      /* 0: start from x = 0 (linearsolve.m:21-23); 1: x holds x0    */
     /* mixed precision (helm3d_solve_mixed) */
-    double inner_tol;   /* relative tolerance of each complex64 inner solve (1e-2)     */
+    double inner_tol;   /* relative tolerance of each complex64 inner solve (1e-2);
+                           0 with BiCGStab: exactly inner_maxit iterations per outer step,
+                           replayed as one captured HIP graph (no host round trips)      */
     int    inner_m;     /* restart length of the inner complex64 GMRES (20)            */
     int    inner_maxit; /* max inner iterations per outer step (200)                   */
+    int    inner_solver;/* HELM3D_INNER_BICGSTAB (default) or HELM3D_INNER_GMRES        */
 } helm3d_solve_opts;
+
+#define HELM3D_INNER_BICGSTAB 0  /* BICGSTAB.m as preconditioner (linearsolve.m:76-77) */
+#define HELM3D_INNER_GMRES    1  /* FGMRES.m(inner_m) as preconditioner (linearsolve.m:81-82) */
 
 typedef struct {
     long long iters;      /* Arnoldi steps (matvecs of the inner loop) of the FP64 loop  */
@@ -177,6 +189,14 @@ int helm3d_solve(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
  * Stops on the FP64 true residual at tol. */
 int helm3d_solve_mixed(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
                        const helm3d_solve_opts *opts, helm3d_solve_info *info);
+
+/* BiCGStab of BICGSTAB.m:37-74 in complex128 (LinSolveOpts.SOLVE_BICGSTAB,
+ * linearsolve.m:133-136), identity preconditioner, x0 = 0.  Stops on the
+ * recursive residual ||r|| / ||b|| < tol or after maxit iterations; info->relres
+ * is the true FP64 residual ||b - A x|| / ||b||.  beta uses the textbook
+ * rho_new / rho_old (BICGSTAB.m:59 divides a dot product by itself). */
+int helm3d_solve_bicgstab(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
+                          const helm3d_solve_opts *opts, helm3d_solve_info *info);
 
 /* ---- FWI gradient ------------------------------------------------------- */
 

@@ -225,3 +225,38 @@ def ls_misfit(x, y):
     """LS_misfit.m:21-26 (no offset mask): f = 1/2 ||x-y||^2, g = x - y."""
     r = x - y
     return 0.5 * float(np.vdot(r, r).real), r
+
+
+def bicgstab(variant: str, wn, h, n, npml, b, tol=1e-6, maxit=10000, nthreads: int = 0):
+    """BiCGStab of BICGSTAB.m:37-74 (identity preconditioner, x0 = 0) on the
+    oracle stencil, numpy vector algebra.  beta = (rho_new/rho_old)(alpha/omega):
+    the textbook update -- BICGSTAB.m:59 divides (rnew, r0_bar) by itself
+    (beta = alpha/omega), a quirk this checker does not reproduce.  Returns
+    (x, iters, relres_history) with iters counting iterations (2 matvecs each)."""
+    A = lambda v: hmvp(variant, wn, h, n, npml, v, nthreads=nthreads)  # noqa: E731
+    b = np.asarray(b, dtype=np.complex128).reshape(-1)
+    x = np.zeros_like(b)
+    r = b.copy()
+    rh = r.copy()
+    p = r.copy()
+    nb = np.linalg.norm(b)
+    rho = np.vdot(rh, r)
+    hist = []
+    it = 0
+    while it < maxit:
+        v = A(p)
+        alpha = rho / np.vdot(rh, v)
+        s = r - alpha * v
+        t = A(s)
+        omega = np.vdot(t, s) / np.vdot(t, t).real
+        x = x + alpha * p + omega * s
+        r = s - omega * t
+        it += 1
+        hist.append(np.linalg.norm(r) / nb)
+        if hist[-1] < tol:
+            break
+        rho_new = np.vdot(rh, r)
+        beta = (rho_new / rho) * (alpha / omega)
+        rho = rho_new
+        p = r + beta * (p - omega * v)
+    return x, it, np.array(hist)

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "bicgstab.hpp"
 #include "helm27.hpp"
 #include "helm3d.h"
 #include "krylov.hpp"
@@ -148,8 +149,17 @@ struct helm3d_ctx {
     size_t small_cap = 0;
     double* small_host = nullptr;  // pinned mirror
 
-    Vecs<double> V64, Z64, T64;
-    Vecs<float> V32, T32a, T32b;
+    Vecs<double> V64, Z64, T64, BC64;
+    Vecs<float> V32, T32a, T32b, BC32;
+    double* bsc = nullptr;  // BiCGStab device scalars (BS_SLOTS doubles)
+    size_t bsc_cap = 0;
+
+    // captured fixed-length complex64 inner solve (mixed precision)
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t inner_graph = nullptr;
+    int inner_graph_variant = -1;
+    long inner_graph_iters = -1;
+    bool use_graphs = true;
 
     // optional per-kernel-class timing with HIP events on `stream`
     bool prof = false;
@@ -162,11 +172,14 @@ struct helm3d_ctx {
     ~helm3d_ctx()
     {
         (void)hipSetDevice(device);
-        void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, V32.p, T32a.p, T32b.p};
+        void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, BC64.p,
+                        V32.p, T32a.p, T32b.p, BC32.p, bsc};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (small_host) (void)hipHostFree(small_host);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (inner_graph) (void)hipGraphExecDestroy(inner_graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
     }
 };
 
@@ -439,6 +452,172 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     return HELM3D_OK;
 }
 
+template <typename R>
+Vecs<R>& bicg_of(helm3d_ctx* c);
+template <>
+Vecs<double>& bicg_of<double>(helm3d_ctx* c) { return c->BC64; }
+template <>
+Vecs<float>& bicg_of<float>(helm3d_ctx* c) { return c->BC32; }
+
+// BiCGStab of BICGSTAB.m:37-74 (identity preconditioner) from z = 0, all
+// scalars on the device; the host reads ||r|| every check_every iterations.
+// Stops when the recursive residual ||r|| / ||b|| < tol (BICGSTAB.m:63) or
+// after maxit iterations (two matvecs each).
+template <typename R>
+int bicgstab_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* z, double tol, long maxit,
+                    int check_every, long* iters_out, double* relres_out)
+{
+    const long N = c->N;
+    const long ld = slot_ld(N);
+    Vecs<R>& W = bicg_of<R>(c);
+    CHK(ensure_vecs(W, N, 6));
+    CHK(ensure_work(c));
+    CHK(ensure_t(&c->bsc, &c->bsc_cap, BS_SLOTS));
+    cpx<R>* r = W.p;
+    cpx<R>* rh = W.p + ld;
+    cpx<R>* p = W.p + 2 * ld;
+    cpx<R>* v = W.p + 3 * ld;
+    cpx<R>* sv = W.p + 4 * ld;
+    cpx<R>* t = W.p + 5 * ld;
+    const OpDesc<R>& op = op_of<R>(c);
+    const int nblk = krylov_blocks(N);
+    double* sc = c->bsc;
+    const cpx<R>* none = nullptr;
+    HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, r, N, false, c->stream)));
+    HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, rh, N, false, c->stream)));
+    HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, p, N, false, c->stream)));
+    HIPCHK((launch_axpby<R, R>(0.0, 0.0, none, 0.0, 0.0, z, N, false, c->stream)));
+    PairArgs<R> P0{{rh, nullptr}, {r, nullptr}};
+    HIPCHK(launch_dots_pairs<R>(P0, 1, true, N, c->part, nblk, c->stream));
+    TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 3, sc + BS_RHO0, c->stream));
+    HIPCHK(hipMemcpyAsync(c->small_host, sc + BS_RHO0 + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double nb = std::sqrt(c->small_host[0]);
+    *iters_out = 0;
+    *relres_out = 0.0;
+    if (nb == 0.0) return HELM3D_OK;
+    double rel = 1.0;
+    long it = 0;
+    for (; it < maxit; ++it) {
+        const int cur = (it & 1) ? BS_RHO1 : BS_RHO0;
+        const int nxt = (it & 1) ? BS_RHO0 : BS_RHO1;
+        CHK(stencil<R>(c, variant, op, p, v, R(1), nullptr, R(0), nullptr));
+        PairArgs<R> P1{{rh, nullptr}, {v, nullptr}};
+        TIMED(HELM3D_PROF_DOTS, launch_dots_pairs<R>(P1, 1, false, N, c->part, nblk, c->stream));
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2, sc + BS_RV, c->stream));
+        TIMED(HELM3D_PROF_UPDATE, launch_bicg_s<R>(r, v, sv, N, sc, cur, c->stream));
+        CHK(stencil<R>(c, variant, op, sv, t, R(1), nullptr, R(0), nullptr));
+        PairArgs<R> P2{{t, nullptr}, {sv, nullptr}};
+        TIMED(HELM3D_PROF_DOTS, launch_dots_pairs<R>(P2, 1, true, N, c->part, nblk, c->stream));
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 3, sc + BS_TS, c->stream));
+        TIMED(HELM3D_PROF_UPDATE, launch_bicg_xr<R>(p, sv, t, rh, z, r, N, sc, c->part, nblk, c->stream));
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 3, sc + nxt, c->stream));
+        if ((it + 1) % check_every == 0 || it + 1 == maxit) {
+            HIPCHK(hipMemcpyAsync(c->small_host, sc + nxt + 2, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->prof) CHK(prof_fold(c));
+            rel = std::sqrt(c->small_host[0]) / nb;
+            if (!(rel >= tol)) {  // converged (or NaN: stop)
+                ++it;
+                break;
+            }
+        }
+        TIMED(HELM3D_PROF_COMBINE, launch_bicg_p<R>(r, v, p, N, sc, nxt, cur, c->stream));
+    }
+    *iters_out = it;
+    *relres_out = rel;
+    return HELM3D_OK;
+}
+
+// Fixed-length BiCGStab (exactly `iters` iterations, no host reads, no
+// profiling events) on stream st: the body captured into a HIP graph.  The
+// breakdown guards of the scalar kernels (zero denominators give zero steps)
+// make a zero right-hand side return z = 0.
+template <typename R>
+int bicgstab_fixed(helm3d_ctx* c, hipStream_t st, int variant, const cpx<R>* b, cpx<R>* z, long iters)
+{
+    const long N = c->N;
+    const long ld = slot_ld(N);
+    cpx<R>* W = bicg_of<R>(c).p;
+    cpx<R>* r = W;
+    cpx<R>* rh = W + ld;
+    cpx<R>* p = W + 2 * ld;
+    cpx<R>* v = W + 3 * ld;
+    cpx<R>* sv = W + 4 * ld;
+    cpx<R>* t = W + 5 * ld;
+    const OpDesc<R>& op = op_of<R>(c);
+    const int nblk = krylov_blocks(N);
+    double* sc = c->bsc;
+    const cpx<R>* none = nullptr;
+    auto sten = [&](const cpx<R>* x, cpx<R>* y) -> int {
+        StencilLaunch<R> L{};
+        L.op = op;
+        L.x = x;
+        L.y = y;
+        L.alpha = R(1);
+        L.sink = static_cast<cpx<R>*>(c->sink);
+        HIPCHK(launch_stencil<R>(variant, L, c->grid, st));
+        return HELM3D_OK;
+    };
+    HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, r, N, false, st)));
+    HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, rh, N, false, st)));
+    HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, p, N, false, st)));
+    HIPCHK((launch_axpby<R, R>(0.0, 0.0, none, 0.0, 0.0, z, N, false, st)));
+    PairArgs<R> P0{{rh, nullptr}, {r, nullptr}};
+    HIPCHK(launch_dots_pairs<R>(P0, 1, true, N, c->part, nblk, st));
+    HIPCHK(launch_reduce(c->part, nblk, 3, sc + BS_RHO0, st));
+    for (long it = 0; it < iters; ++it) {
+        const int cur = (it & 1) ? BS_RHO1 : BS_RHO0;
+        const int nxt = (it & 1) ? BS_RHO0 : BS_RHO1;
+        CHK(sten(p, v));
+        PairArgs<R> P1{{rh, nullptr}, {v, nullptr}};
+        HIPCHK(launch_dots_pairs<R>(P1, 1, false, N, c->part, nblk, st));
+        HIPCHK(launch_reduce(c->part, nblk, 2, sc + BS_RV, st));
+        HIPCHK(launch_bicg_s<R>(r, v, sv, N, sc, cur, st));
+        CHK(sten(sv, t));
+        PairArgs<R> P2{{t, nullptr}, {sv, nullptr}};
+        HIPCHK(launch_dots_pairs<R>(P2, 1, true, N, c->part, nblk, st));
+        HIPCHK(launch_reduce(c->part, nblk, 3, sc + BS_TS, st));
+        HIPCHK(launch_bicg_xr<R>(p, sv, t, rh, z, r, N, sc, c->part, nblk, st));
+        HIPCHK(launch_reduce(c->part, nblk, 3, sc + nxt, st));
+        if (it + 1 < iters) HIPCHK(launch_bicg_p<R>(r, v, p, N, sc, nxt, cur, st));
+    }
+    return HELM3D_OK;
+}
+
+// Run the fixed-length complex64 inner solve T32b = BiCGStab(T32a) on the
+// context stream, replaying a captured graph (captured on the first call for
+// this variant and length).
+int inner_fixed_c64(helm3d_ctx* c, int variant, long iters)
+{
+    CHK(ensure_vecs(c->BC32, c->N, 6));
+    CHK(ensure_t(&c->bsc, &c->bsc_cap, BS_SLOTS));
+    if (!c->use_graphs) return bicgstab_fixed<float>(c, c->stream, variant, c->T32a.p, c->T32b.p, iters);
+    if (!c->inner_graph || c->inner_graph_variant != variant || c->inner_graph_iters != iters) {
+        if (c->inner_graph) {
+            HIPCHK(hipGraphExecDestroy(c->inner_graph));
+            c->inner_graph = nullptr;
+        }
+        if (!c->cap_stream) HIPCHK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int st = bicgstab_fixed<float>(c, c->cap_stream, variant, c->T32a.p, c->T32b.p, iters);
+        const hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+        if (st != HELM3D_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIPCHK(e);
+        const hipError_t ei = hipGraphInstantiate(&c->inner_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+        c->inner_graph_variant = variant;
+        c->inner_graph_iters = iters;
+    }
+    HIPCHK(hipGraphLaunch(c->inner_graph, c->stream));
+    return HELM3D_OK;
+}
+
 // complex64 inner solve as the flexible preconditioner of the FP64 outer loop
 struct MixedPrecond : Precond {
     helm3d_ctx* c;
@@ -446,16 +625,28 @@ struct MixedPrecond : Precond {
     double inner_tol;
     int inner_m;
     long inner_cap;
+    int inner_solver = HELM3D_INNER_BICGSTAB;
     long inner_total = 0;
     int apply(const cpx<double>* v_raw, double s, cpx<double>* z) override
     {
         const long N = c->N;
         // T32a = c64(s * v), solve A t = T32a in complex64, z = c128(t)
         HIPCHK((launch_axpby<double, float>(s, 0.0, v_raw, 0.0, 0.0, c->T32a.p, N, false, c->stream)));
-        GmresResult r;
-        CHK(gmres_engine<float>(c, variant, c->T32a.p, c->T32b.p, false, inner_tol, inner_m, 1L << 40, inner_cap,
-                                nullptr, &r, nullptr));
-        inner_total += r.iters;
+        if (inner_solver == HELM3D_INNER_BICGSTAB && inner_tol <= 0.0) {
+            // fixed number of iterations: one captured graph per outer step
+            CHK(inner_fixed_c64(c, variant, inner_cap));
+            inner_total += 2 * inner_cap;
+        } else if (inner_solver == HELM3D_INNER_BICGSTAB) {
+            long its = 0;
+            double rel = 0;
+            CHK(bicgstab_engine<float>(c, variant, c->T32a.p, c->T32b.p, inner_tol, inner_cap, 4, &its, &rel));
+            inner_total += 2 * its;  // matvecs
+        } else {
+            GmresResult r;
+            CHK(gmres_engine<float>(c, variant, c->T32a.p, c->T32b.p, false, inner_tol, inner_m, 1L << 40,
+                                    inner_cap, nullptr, &r, nullptr));
+            inner_total += r.iters;
+        }
         HIPCHK((launch_axpby<float, double>(1.0, 0.0, c->T32b.p, 0.0, 0.0, z, N, false, c->stream)));
         return HELM3D_OK;
     }
@@ -656,6 +847,7 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
     CHK(ensure_work(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_model = true;
+    c->inner_graph_iters = -1;  // captured graphs refer to the old model's buffers
     return HELM3D_OK;
 }
 
@@ -666,7 +858,20 @@ int helm3d_set_launch(helm3d_ctx* c, int kchunk, int tile_rows, int prefetch)
         return fail(HELM3D_ERR_ARG, "tile_rows in {4,8}, prefetch in {1,2}, kchunk >= 0");
     c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kchunk, tile_rows, prefetch);
     CHK(ensure_work(c));
+    c->inner_graph_iters = -1;
     return HELM3D_OK;
+}
+
+int helm3d_set_option(helm3d_ctx* c, int option, long long value)
+{
+    CHK(bind(c));
+    switch (option) {
+    case HELM3D_OPT_GRAPHS:
+        c->use_graphs = value != 0;
+        return HELM3D_OK;
+    default:
+        return fail(HELM3D_ERR_ARG, "unknown option");
+    }
 }
 
 int helm3d_get_launch(helm3d_ctx* c, int* kchunk, int* tile_rows, int* prefetch, int* blocks)
@@ -725,6 +930,40 @@ void helm3d_solve_opts_default(helm3d_solve_opts* o)
     o->inner_tol = 1e-2;
     o->inner_m = 20;
     o->inner_maxit = 200;
+    o->inner_solver = HELM3D_INNER_BICGSTAB;
+}
+
+int helm3d_solve_bicgstab(helm3d_ctx* c, int mode, const void* b, void* x, const helm3d_solve_opts* opts,
+                          helm3d_solve_info* info)
+{
+    CHK(check_model(c));
+    if (!b || !x) return fail(HELM3D_ERR_ARG, "null vector");
+    helm3d_solve_opts o;
+    helm3d_solve_opts_default(&o);
+    if (opts) o = *opts;
+    const int variant = model_variant(c, mode != 1);
+    long its = 0;
+    double rel = 0;
+    auto* bb = static_cast<const cpx<double>*>(b);
+    auto* xx = static_cast<cpx<double>*>(x);
+    CHK(bicgstab_engine<double>(c, variant, bb, xx, o.tol, long(o.maxit), 1, &its, &rel));
+    // true residual ||b - A x|| / ||b|| in FP64
+    CHK(ensure_vecs(c->T64, c->N, 1));
+    CHK(stencil<double>(c, variant, c->op64, xx, c->T64.p, -1.0, bb, 1.0, c->part));
+    CHK(reduce_to_small(c, c->grid.blocks, 1, 0));
+    HIPCHK(launch_norm2<double>(bb, c->N, c->part, krylov_blocks(c->N), c->stream));
+    CHK(reduce_to_small(c, krylov_blocks(c->N), 1, 1));
+    CHK(fetch_small(c, 2));
+    const double relres = c->small_host[1] > 0 ? std::sqrt(c->small_host[0] / c->small_host[1]) : 0.0;
+    if (info) {
+        info->iters = 2 * its;
+        info->inner_iters = 0;
+        info->cycles = its;
+        info->relres = relres;
+        info->quasi_relres = rel;
+        info->converged = rel < o.tol;
+    }
+    return rel < o.tol ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
 }
 
 int helm3d_solve(helm3d_ctx* c, int mode, const void* b, void* x, const helm3d_solve_opts* opts,
@@ -772,6 +1011,7 @@ int helm3d_solve_mixed(helm3d_ctx* c, int mode, const void* b, void* x, const he
     P.inner_tol = o.inner_tol;
     P.inner_m = o.inner_m;
     P.inner_cap = o.inner_maxit;
+    P.inner_solver = o.inner_solver;
     GmresResult r;
     CHK(gmres_engine<double>(c, P.variant, static_cast<const cpx<double>*>(b), static_cast<cpx<double>*>(x),
                              o.use_x0 != 0, o.tol, o.maxinnerit, long(o.maxit), 0, &P, &r, nullptr));

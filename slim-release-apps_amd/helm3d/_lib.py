@@ -33,6 +33,8 @@ HELM3D_WNC = 4
 HELM3D_C128 = 0
 HELM3D_C64 = 1
 
+HELM3D_OPT_GRAPHS = 1
+
 # MEX error identifiers of Helm3dmvp_mex.c:77,83 (the Xsize id names the sweep
 # gateway in the reference too).
 MEX_ERR_IDS = {
@@ -48,7 +50,11 @@ _vp = ctypes.c_void_p
 class SolveOpts(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("maxinnerit", ctypes.c_int), ("maxit", ctypes.c_longlong),
                 ("use_x0", ctypes.c_int), ("inner_tol", ctypes.c_double), ("inner_m", ctypes.c_int),
-                ("inner_maxit", ctypes.c_int)]
+                ("inner_maxit", ctypes.c_int), ("inner_solver", ctypes.c_int)]
+
+
+HELM3D_INNER_BICGSTAB = 0
+HELM3D_INNER_GMRES = 1
 
 
 class SolveInfo(ctypes.Structure):
@@ -84,8 +90,11 @@ _SIGS = [
                                     ctypes.POINTER(ctypes.c_longlong)]),
     ("helm3d_solve_mixed", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
                                           ctypes.POINTER(SolveInfo)]),
+    ("helm3d_solve_bicgstab", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
+                                             ctypes.POINTER(SolveInfo)]),
     ("helm3d_fwi_grad_accum", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
     ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
     ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
     ("helm3d_profile", ctypes.c_int, [_vp, ctypes.c_int]),
     ("helm3d_profile_read", ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]),

@@ -154,7 +154,7 @@ class Helm3d:
 
     # -- solves -------------------------------------------------------------
     @staticmethod
-    def _opts(tol, maxinnerit, maxit, use_x0, inner_tol=1e-2, inner_m=20, inner_maxit=200):
+    def _opts(tol, maxinnerit, maxit, use_x0, inner_tol=1e-2, inner_m=20, inner_maxit=200, inner="bicgstab"):
         o = _lib.SolveOpts()
         lib().helm3d_solve_opts_default(ctypes.byref(o))
         o.tol = float(tol)
@@ -164,6 +164,7 @@ class Helm3d:
         o.inner_tol = float(inner_tol)
         o.inner_m = int(inner_m)
         o.inner_maxit = int(inner_maxit)
+        o.inner_solver = {"bicgstab": _lib.HELM3D_INNER_BICGSTAB, "gmres": _lib.HELM3D_INNER_GMRES}[inner]
         return o
 
     def solve(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxinnerit: int = 20,
@@ -194,11 +195,14 @@ class Helm3d:
 
     def solve_mixed(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxinnerit: int = 20,
                     maxit: int = 10000, inner_tol: float = 1e-2, inner_m: int = 20, inner_maxit: int = 200,
-                    x0: Optional[torch.Tensor] = None, raise_on_notconv: bool = False):
-        """FP64 flexible GMRES with a complex64 GMRES preconditioner (mixed precision)."""
+                    x0: Optional[torch.Tensor] = None, raise_on_notconv: bool = False, inner: str = "bicgstab"):
+        """Mixed precision: FP64 flexible GMRES (FGMRES.m with a preconditioner)
+        whose preconditioner is a complex64 solve of the same operator to
+        inner_tol -- BiCGStab (default, linearsolve.m:76-77) or GMRES(inner_m)
+        (linearsolve.m:81-82).  inner_iters counts complex64 matvecs."""
         b = self._vec(b, torch.complex128)
         x = torch.zeros_like(b) if x0 is None else self._vec(x0, torch.complex128).clone()
-        o = self._opts(tol, maxinnerit, maxit, x0 is not None, inner_tol, inner_m, inner_maxit)
+        o = self._opts(tol, maxinnerit, maxit, x0 is not None, inner_tol, inner_m, inner_maxit, inner)
         info = _lib.SolveInfo()
         with torch.cuda.device(self.device):
             self._bind()
@@ -207,6 +211,27 @@ class Helm3d:
         check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
         return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
                             bool(info.converged), inner_iters=int(info.inner_iters))
+
+    def solve_bicgstab(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxit: int = 10000,
+                       raise_on_notconv: bool = False):
+        """BiCGStab (BICGSTAB.m, LinSolveOpts.SOLVE_BICGSTAB) in complex128 from
+        x = 0; info.iters counts matvecs, info.relres is the true FP64 residual."""
+        b = self._vec(b, torch.complex128)
+        x = torch.zeros_like(b)
+        o = self._opts(tol, 20, maxit, False)
+        info = _lib.SolveInfo()
+        with torch.cuda.device(self.device):
+            self._bind()
+            code = lib().helm3d_solve_bicgstab(self._ctx, int(mode), ctypes.c_void_p(_dptr(b)),
+                                               ctypes.c_void_p(_dptr(x)), ctypes.byref(o), ctypes.byref(info))
+        check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
+        return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
+                            bool(info.converged))
+
+    def set_graphs(self, enable: bool):
+        """Replay fixed kernel sequences from captured HIP graphs (default on)."""
+        with torch.cuda.device(self.device):
+            check(lib().helm3d_set_option(self._ctx, _lib.HELM3D_OPT_GRAPHS, int(enable)))
 
     # -- launch shape ---------------------------------------------------------
     def set_launch(self, kchunk: int = 0, tile_rows: int = 4, prefetch: int = 1):

@@ -73,7 +73,27 @@ def test_solve_zero_rhs_and_x0():
     assert abs(i2.iters - io2["iters"]) <= 0.02 * io2["iters"] + 2
 
 
-def test_mixed_precision_solve():
+def test_bicgstab_fp64_matches_oracle():
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small
+    cfg = config_small(n=(24, 24, 20), npml=6)
+    wn, b = _setup(cfg)
+    h3, p6 = (cfg.h,) * 3, np.array(cfg.npml6)
+    xo, it_o, hist = O.bicgstab("forw", wn, h3, cfg.nt, p6, b, tol=1e-6)
+    dev = torch.device("cuda:0")
+    H = Helm3d(cfg.nt, h3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    x, info = H.solve_bicgstab(torch.from_numpy(b).to(dev), tol=1e-6)
+    print("bicgstab gpu matvecs", info.iters, "oracle iters", it_o, info.relres)
+    assert info.converged
+    assert abs(info.iters // 2 - it_o) <= 0.05 * it_o + 2
+    r = b - O.hmvp("forw", wn, h3, cfg.nt, p6, x.cpu().numpy())
+    assert abs(np.linalg.norm(r) / np.linalg.norm(b) - info.relres) < 1e-9
+    assert info.relres < 1e-5
+
+
+@pytest.mark.parametrize("inner", ["bicgstab", "gmres"])
+def test_mixed_precision_solve(inner):
     import torch
     from helm3d import Helm3d
     from helm3d.models import config_small
@@ -83,7 +103,43 @@ def test_mixed_precision_solve():
     H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
     bt = torch.from_numpy(b).to(dev)
     x64, i64 = H.solve(bt, tol=1e-6)
-    xm, im = H.solve_mixed(bt, tol=1e-6, inner_tol=1e-2)
-    print("plain", i64.iters, "mixed outer", im.iters, "inner", im.inner_iters, im.relres)
+    xm, im = H.solve_mixed(bt, tol=1e-6, inner_tol=1e-2, inner=inner)
+    print(inner, "plain", i64.iters, "mixed outer", im.iters, "inner", im.inner_iters, im.relres)
     r = b - O.hmvp("forw", wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), xm.cpu().numpy())
     assert im.converged and np.linalg.norm(r) / np.linalg.norm(b) < 1e-6
+    # per-solve error against the complex128 solution, both at residual 1e-6
+    err = float(torch.linalg.norm(xm - x64) / torch.linalg.norm(x64))
+    print("error vs c128 solve", err)
+    assert err < 1e-3
+    # adjoint system
+    xa, ia = H.solve_mixed(bt, mode=2, tol=1e-6, inner=inner)
+    ra = b - O.hmvp("adj", wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), xa.cpu().numpy())
+    assert ia.converged and np.linalg.norm(ra) / np.linalg.norm(b) < 1e-6
+
+
+def test_mixed_fixed_inner_graph_replays_match_eager():
+    """Fixed-length complex64 inner solves replayed from a captured HIP graph
+    give bit-identical solutions to eager launches, over repeated replays with
+    eager work in between."""
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small
+    cfg = config_small(n=(24, 24, 20), npml=6)
+    wn, b = _setup(cfg)
+    dev = torch.device("cuda:0")
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    bt = torch.from_numpy(b).to(dev)
+    H.set_graphs(False)
+    xe, ie = H.solve_mixed(bt, tol=1e-6, inner_tol=0.0, inner_maxit=10)
+    H.set_graphs(True)
+    for rep in range(3):
+        xg, ig = H.solve_mixed(bt, tol=1e-6, inner_tol=0.0, inner_maxit=10)
+        assert ig.iters == ie.iters and torch.equal(xg, xe), rep
+        H.apply(bt)  # eager work between replays
+        torch.cuda.synchronize()
+    r = b - O.hmvp("forw", wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), xg.cpu().numpy())
+    assert ig.converged and np.linalg.norm(r) / np.linalg.norm(b) < 1e-6
+    # the adjoint re-captures (different operator variant)
+    xa, ia = H.solve_mixed(bt, mode=2, tol=1e-6, inner_tol=0.0, inner_maxit=10)
+    ra = b - O.hmvp("adj", wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), xa.cpu().numpy())
+    assert ia.converged and np.linalg.norm(ra) / np.linalg.norm(b) < 1e-6
