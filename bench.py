@@ -107,6 +107,52 @@ def cpu_baseline(cfg, wn, b, gpu_iters, sample_iters):
     }
 
 
+def copy_mode(args):
+    """K copies of a 256 MiB buffer: the known-size kernel that calibrates the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"mode": "copy", "bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                      "ms_per_launch": dt * 1e3, "GB/s": 2 * nbytes / dt / 1e9}))
+
+
+def matvec_mode(args, cfg, wn):
+    """K FP64 forward matvecs y = H x on the benchmark grid (40 B/pt)."""
+    import numpy as np
+    import torch
+    from helm3d import Helm3d
+    dev = torch.device("cuda:0")
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    x = torch.from_numpy(np.random.default_rng(0).standard_normal(H.N) + 0j).to(dev)
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        H.apply(x, out=y)
+    torch.cuda.synchronize()
+    H.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        H.apply(x, out=y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ms, n = H.profile_read()["stencil"]
+    H.profile(False)
+    kern = ms / max(1, n)
+    gbs = 40.0 * H.N / (kern * 1e-3) / 1e9
+    print(json.dumps({"mode": "matvec", "grid_points": H.N, "alg_bytes_per_launch": 40.0 * H.N,
+                      "kernel_ms": kern, "wall_ms_per_launch": dt * 1e3, "GB/s": gbs,
+                      "pct_hbm_peak": 100.0 * gbs / HBM_PEAK_GBS, "launch": H.get_launch()}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +160,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
+    ap.add_argument("--mode", choices=("solve", "matvec", "copy"), default="solve",
+                    help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
+                         "(profiling workload); copy: K device copies of 256 MiB (PMC calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -128,8 +177,12 @@ def main():
     from helm3d import Helm3d
     from helm3d.models import config_c2, extend, wavenumber, vec, point_source
 
+    if args.mode == "copy":
+        return copy_mode(args)
     cfg = config_c2()
     wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    if args.mode == "matvec":
+        return matvec_mode(args, cfg, wn)
     b = point_source(cfg, cfg.sources[0])
     dev = torch.device("cuda:0")
     H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))

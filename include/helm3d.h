@@ -200,15 +200,54 @@ int helm3d_solve_bicgstab(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_
 
 /* ---- FWI gradient ------------------------------------------------------- */
 
-/* g[phys] += scale * Re( conj(U) .* (dH^H V) )[pml grid -> physical grid]:
- * the gradient term sum_srcs(T(U)'*V) of PDEfunc.m:393-394 with
- * T_adj(z,u) = conj(u).*(dH'*z) (discrete_helmholtz.m:288), dH the DERIV
- * stencil with real dwn (device, N doubles), and comp2phys the truncation of
- * the PML (discrete_helmholtz.m:231-233).  U, V: device complex128 vectors on
- * the PML grid; g: device doubles on the physical grid
- * (n - npml_lo - npml_hi per axis). */
+#define HELM3D_GRID_PHYS 0  /* g on the physical grid: comp2phys = truncation  */
+#define HELM3D_GRID_PML  1  /* g on the whole PML-extended grid               */
+
+/* g += scale * Re( conj(U) .* (dH^H V) ): the gradient term sum_srcs(T(U)'*V)
+ * of PDEfunc.m:393-394 with T_adj(z,u) = conj(u).*(dH'*z)
+ * (discrete_helmholtz.m:288) and dH the DERIV stencil with real dwn (device,
+ * N doubles; NULL means dwn = 1, so that a constant dwn -- s2/m2 units -- is
+ * passed as scale).  U, V: device complex128 vectors on the PML grid.
+ * grid HELM3D_GRID_PHYS: g holds the physical grid (n - npml_lo - npml_hi per
+ * axis) and receives the truncated term, i.e. comp2phys = Ppad' of
+ * discrete_helmholtz.m:231-233 (helm_cut_pml, the default).
+ * grid HELM3D_GRID_PML: g holds the extended grid (N doubles); fold it with
+ * helm3d_extension_adjoint for comp2phys = Pext' (:234-235).  One stencil pass:
+ * reads V, U, g (and dwn), writes g. */
 int helm3d_fwi_grad_accum(helm3d_ctx *ctx, const double *dwn_dev, const void *U_dev,
-                          const void *V_dev, double scale, double *g_dev);
+                          const void *V_dev, double scale, int grid, double *g_dev);
+
+/* ---- model set-up and source / receiver operators -------------------------- */
+
+#define HELM3D_UNIT_MS    0  /* 'm/s'    */
+#define HELM3D_UNIT_S2M2  1  /* 's2/m2'  */
+#define HELM3D_UNIT_S2KM2 2  /* 's2/km2' */
+
+/* The model v (device, physical grid n_phys, x fastest) extended into the PML
+ * by edge replication (Pext = phys2comp, discrete_helmholtz.m:214-217,240) and
+ * mapped to the wavenumber parameter and its derivatives,
+ * [wn, dwn, ddwn] = param2wavenum(Pext v, freq, unit) (param2wavenum.m:20-49).
+ * Outputs: device doubles on the extended grid; dwn and ddwn may be NULL.
+ * Runs on the context's stream and needs no model. */
+int helm3d_param2wavenum(helm3d_ctx *ctx, const int n_phys[3], const int npml[6],
+                         const double *v_dev, int unit, double freq, double *wn_dev,
+                         double *dwn_dev, double *ddwn_dev);
+
+/* g = Pext' gpml (accumulate: g += Pext' gpml): the adjoint of the edge
+ * replication, opExtension.m:97-101 composed per axis.  gpml: extended grid,
+ * g: physical grid (device doubles). */
+int helm3d_extension_adjoint(helm3d_ctx *ctx, const int n_phys[3], const int npml[6],
+                             const double *gpml_dev, double *g_dev, int accumulate);
+
+/* Row-compressed real-weight operator on complex128 device vectors, the form
+ * of the source injection Ps and receiver sampling Pr operators
+ * (DataGrid.m:22-44, opInterp.m:34-57):
+ *   out[oidx ? oidx[r] : r] (accumulate ? += : =) alpha * sum_e w[e] in[col[e]],
+ *   e in [rowptr[r], rowptr[r+1]),  r < nrows.
+ * All index and weight arrays are device memory; oidx rows must be distinct. */
+int helm3d_csr_apply(helm3d_ctx *ctx, int nrows, const long long *rowptr_dev,
+                     const long long *col_dev, const double *w_dev, const long long *oidx_dev,
+                     double alpha, const void *in_dev, void *out_dev, int accumulate);
 
 /* ---- in-library kernel timing ------------------------------------------ *
  * When enabled, every kernel the context launches is bracketed by HIP events

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "bicgstab.hpp"
+#include "fwi.hpp"
 #include "helm27.hpp"
 #include "helm3d.h"
 #include "krylov.hpp"
@@ -1026,11 +1027,12 @@ int helm3d_solve_mixed(helm3d_ctx* c, int mode, const void* b, void* x, const he
     return r.converged ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
 }
 
-int helm3d_fwi_grad_accum(helm3d_ctx* c, const double* dwn, const void* U, const void* V, double scale,
+int helm3d_fwi_grad_accum(helm3d_ctx* c, const double* dwn, const void* U, const void* V, double scale, int grid,
                           double* g)
 {
     CHK(check_model(c));
     if (!U || !V || !g) return fail(HELM3D_ERR_ARG, "null argument");
+    if (grid != HELM3D_GRID_PHYS && grid != HELM3D_GRID_PML) return fail(HELM3D_ERR_ARG, "bad grid");
     StencilLaunch<double> L{};
     L.op = c->op64;
     L.op.wn = dwn;
@@ -1044,11 +1046,70 @@ int helm3d_fwi_grad_accum(helm3d_ctx* c, const double* dwn, const void* U, const
     L.sink = static_cast<cpx<double>*>(c->sink);
     L.gscale = scale;
     for (int a = 0; a < 3; ++a) {
-        L.p0[a] = c->npml[2 * a];
-        L.pn[a] = c->n[a] - c->npml[2 * a] - c->npml[2 * a + 1];
+        const bool phys = grid == HELM3D_GRID_PHYS;
+        L.p0[a] = phys ? c->npml[2 * a] : 0;
+        L.pn[a] = phys ? c->n[a] - c->npml[2 * a] - c->npml[2 * a + 1] : c->n[a];
         if (L.pn[a] < 1) return fail(HELM3D_ERR_ARG, "PML covers the whole axis");
     }
     HIPCHK(launch_stencil<double>(VAR_ADJ | VAR_DERIV, L, c->grid, c->stream));
+    return HELM3D_OK;
+}
+
+namespace {
+
+int ext_grid(const int n_phys[3], const int npml[6], ExtGrid* G)
+{
+    if (!n_phys || !npml) return fail(HELM3D_ERR_ARG, "null argument");
+    for (int a = 0; a < 3; ++a) {
+        if (n_phys[a] < 1 || npml[2 * a] < 0 || npml[2 * a + 1] < 0)
+            return fail(HELM3D_ERR_ARG, "bad grid sizes");
+        G->np[a] = n_phys[a];
+        G->lo[a] = npml[2 * a];
+        const long nt = long(n_phys[a]) + npml[2 * a] + npml[2 * a + 1];
+        if (nt > 1000000) return fail(HELM3D_ERR_ARG, "grid too large");
+        G->nt[a] = int(nt);
+    }
+    return HELM3D_OK;
+}
+
+}  // namespace
+
+int helm3d_param2wavenum(helm3d_ctx* c, const int n_phys[3], const int npml[6], const double* v, int unit,
+                         double freq, double* wn, double* dwn, double* ddwn)
+{
+    if (!c || !v || !wn) return fail(HELM3D_ERR_ARG, "null argument");
+    if (unit != HELM3D_UNIT_MS && unit != HELM3D_UNIT_S2M2 && unit != HELM3D_UNIT_S2KM2)
+        return fail(HELM3D_ERR_ARG, "Unknown unit");
+    ExtGrid G;
+    CHK(ext_grid(n_phys, npml, &G));
+    CHK(bind(c));
+    const double w = 2.0 * M_PI * freq;
+    double cc = w * w;
+    if (unit == HELM3D_UNIT_S2KM2) cc *= 1e-6;
+    HIPCHK(launch_param2wavenum(G, v, unit == HELM3D_UNIT_MS ? 0 : 1, cc, wn, dwn, ddwn, c->stream));
+    return HELM3D_OK;
+}
+
+int helm3d_extension_adjoint(helm3d_ctx* c, const int n_phys[3], const int npml[6], const double* gpml,
+                             double* g, int accumulate)
+{
+    if (!c || !gpml || !g) return fail(HELM3D_ERR_ARG, "null argument");
+    ExtGrid G;
+    CHK(ext_grid(n_phys, npml, &G));
+    CHK(bind(c));
+    HIPCHK(launch_extension_adjoint(G, gpml, g, accumulate != 0, c->stream));
+    return HELM3D_OK;
+}
+
+int helm3d_csr_apply(helm3d_ctx* c, int nrows, const long long* rowptr, const long long* col, const double* w,
+                     const long long* oidx, double alpha, const void* in, void* out, int accumulate)
+{
+    if (!c || nrows < 0) return fail(HELM3D_ERR_ARG, "bad argument");
+    if (nrows == 0) return HELM3D_OK;
+    if (!rowptr || !col || !w || !in || !out) return fail(HELM3D_ERR_ARG, "null argument");
+    CHK(bind(c));
+    HIPCHK(launch_csr_apply(nrows, rowptr, col, w, oidx, alpha, static_cast<const cpx<double>*>(in),
+                            static_cast<cpx<double>*>(out), accumulate != 0, c->stream));
     return HELM3D_OK;
 }
 

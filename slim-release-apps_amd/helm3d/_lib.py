@@ -35,6 +35,11 @@ HELM3D_C64 = 1
 
 HELM3D_OPT_GRAPHS = 1
 
+HELM3D_GRID_PHYS = 0
+HELM3D_GRID_PML = 1
+
+HELM3D_UNITS = {"m/s": 0, "s2/m2": 1, "s2/km2": 2}
+
 # MEX error identifiers of Helm3dmvp_mex.c:77,83 (the Xsize id names the sweep
 # gateway in the reference too).
 MEX_ERR_IDS = {
@@ -92,7 +97,11 @@ _SIGS = [
                                           ctypes.POINTER(SolveInfo)]),
     ("helm3d_solve_bicgstab", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
                                              ctypes.POINTER(SolveInfo)]),
-    ("helm3d_fwi_grad_accum", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, _vp]),
+    ("helm3d_fwi_grad_accum", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, _vp]),
+    ("helm3d_param2wavenum", ctypes.c_int, [_vp, _ip, _ip, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]),
+    ("helm3d_extension_adjoint", ctypes.c_int, [_vp, _ip, _ip, _vp, _vp, ctypes.c_int]),
+    ("helm3d_csr_apply", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp,
+                                        ctypes.c_int]),
     ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
     ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),

@@ -61,9 +61,9 @@ class SolveInfo:
 class Helm3d:
     """H(wn) on one device; forward / adjoint / derivative products and solves."""
 
-    def __init__(self, n, h, npml, wn: torch.Tensor, device=None):
+    def __init__(self, n=None, h=None, npml=None, wn: Optional[torch.Tensor] = None, device=None):
         if device is None:
-            device = wn.device
+            device = wn.device if wn is not None else torch.device("cuda", torch.cuda.current_device())
         device = torch.device(device)
         if device.type != "cuda":
             raise RuntimeError("Helm3d runs on a HIP device; got " + str(device))
@@ -72,7 +72,9 @@ class Helm3d:
         with torch.cuda.device(device):
             check(lib().helm3d_create(device.index if device.index is not None else -1,
                                       ctypes.c_void_p(_stream_ptr()), ctypes.byref(self._ctx)))
-        self.set_model(n, h, npml, wn)
+        self.n = None
+        if wn is not None:
+            self.set_model(n, h, npml, wn)
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -266,21 +268,119 @@ class Helm3d:
 
     # -- gradient -----------------------------------------------------------
     def grad_accum(self, U: torch.Tensor, V: torch.Tensor, g: torch.Tensor, scale: float = 1.0,
-                   dwn: Optional[torch.Tensor] = None):
-        """g += scale * comp2phys(Re(conj(U) .* (dH^H V))) (PDEfunc.m:394)."""
+                   dwn: Optional[torch.Tensor] = None, grid: str = "phys"):
+        """g += scale * Re(conj(U) .* (dH^H V)) (PDEfunc.m:394 before comp2phys).
+        grid "phys": g on the physical grid, PML truncated (cut_pml);
+        grid "pml": g on the extended grid (fold with extension_adjoint)."""
         U = self._vec(U, torch.complex128)
         V = self._vec(V, torch.complex128)
-        if g.dtype != torch.float64 or not g.is_contiguous() or g.numel() != int(np.prod(self.n_phys)):
-            raise ValueError("g must be a contiguous float64 tensor on the physical grid")
+        gcode = {"phys": _lib.HELM3D_GRID_PHYS, "pml": _lib.HELM3D_GRID_PML}[grid]
+        size = int(np.prod(self.n_phys)) if gcode == _lib.HELM3D_GRID_PHYS else self.N
+        if (g.dtype != torch.float64 or not g.is_contiguous() or g.numel() != size or g.device != self.device):
+            raise ValueError(f"g must be a contiguous float64 tensor of {size} elements on {self.device}")
         d = None
         if dwn is not None:
             d = dwn.reshape(-1).to(device=self.device, dtype=torch.float64).contiguous()
+            if d.numel() != self.N:
+                raise ValueError("dwn must have prod(n) elements")
         with torch.cuda.device(self.device):
             self._bind()
             check(lib().helm3d_fwi_grad_accum(self._ctx, ctypes.c_void_p(_dptr(d)) if d is not None else None,
                                               ctypes.c_void_p(_dptr(U)), ctypes.c_void_p(_dptr(V)),
-                                              float(scale), ctypes.c_void_p(_dptr(g))))
+                                              float(scale), gcode, ctypes.c_void_p(_dptr(g))))
         return g
+
+    # -- model set-up and sparse source / receiver operators ------------------
+    def param2wavenum(self, v: torch.Tensor, n_phys, npml, freq: float, unit: str = "m/s",
+                      derivs: int = 1):
+        """[wn, dwn, ddwn] = param2wavenum(Pext v, freq, unit) on the extended
+        grid (discrete_helmholtz.m:240-244, param2wavenum.m:20-49).  Returns
+        (wn, dwn or None, ddwn or None) as float64 device tensors."""
+        n_phys = _as_int3(n_phys)
+        p6 = npml_flat(npml)
+        if unit not in _lib.HELM3D_UNITS:
+            raise ValueError("Unknown unit - " + str(unit))
+        v = v.reshape(-1).to(device=self.device, dtype=torch.float64).contiguous()
+        if v.numel() != int(np.prod(n_phys)):
+            raise ValueError("v must have prod(n_phys) elements")
+        nt = tuple(n_phys[a] + p6[2 * a] + p6[2 * a + 1] for a in range(3))
+        N = int(np.prod(nt))
+        wn = torch.empty(N, dtype=torch.float64, device=self.device)
+        dwn = torch.empty_like(wn) if derivs >= 1 else None
+        ddwn = torch.empty_like(wn) if derivs >= 2 else None
+        P = lambda t: ctypes.c_void_p(_dptr(t)) if t is not None else None  # noqa: E731
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_param2wavenum(self._ctx, (ctypes.c_int * 3)(*n_phys), (ctypes.c_int * 6)(*p6),
+                                             P(v), _lib.HELM3D_UNITS[unit], float(freq), P(wn), P(dwn), P(ddwn)))
+        return wn, dwn, ddwn
+
+    def extend(self, t: torch.Tensor, n_phys, npml) -> torch.Tensor:
+        """Pext t: edge replication of a physical-grid field into the PML
+        (opExtension.m:94-96); param2wavenum with c = (2 pi f)^2 = 1, s2/m2."""
+        return self.param2wavenum(t, n_phys, npml, 1.0 / (2.0 * np.pi), "s2/m2", derivs=0)[0]
+
+    def extension_adjoint(self, gpml: torch.Tensor, n_phys, npml, g: Optional[torch.Tensor] = None,
+                          accumulate: bool = False) -> torch.Tensor:
+        """g (+)= Pext' gpml: adjoint of the edge replication (opExtension.m:97-101)."""
+        n_phys = _as_int3(n_phys)
+        p6 = npml_flat(npml)
+        nt = tuple(n_phys[a] + p6[2 * a] + p6[2 * a + 1] for a in range(3))
+        if gpml.dtype != torch.float64 or not gpml.is_contiguous() or gpml.numel() != int(np.prod(nt)):
+            raise ValueError("gpml must be a contiguous float64 tensor on the extended grid")
+        if g is None:
+            g = torch.empty(int(np.prod(n_phys)), dtype=torch.float64, device=self.device)
+            accumulate = False
+        elif g.dtype != torch.float64 or not g.is_contiguous() or g.numel() != int(np.prod(n_phys)):
+            raise ValueError("g must be a contiguous float64 tensor on the physical grid")
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_extension_adjoint(self._ctx, (ctypes.c_int * 3)(*n_phys), (ctypes.c_int * 6)(*p6),
+                                                 ctypes.c_void_p(_dptr(gpml)), ctypes.c_void_p(_dptr(g)),
+                                                 int(accumulate)))
+        return g
+
+    def csr_apply(self, op: "SparseRows", x: torch.Tensor, out: torch.Tensor, alpha: float = 1.0,
+                  accumulate: bool = False) -> torch.Tensor:
+        """out[rows] (+)= alpha * (op x)[rows] for a SparseRows operator; rows of
+        op without entries are left untouched."""
+        if x.dtype != torch.complex128 or out.dtype != torch.complex128:
+            raise ValueError("csr_apply works on complex128 vectors")
+        if x.numel() != op.shape[1] or out.numel() != op.shape[0]:
+            raise ValueError(f"shape mismatch: op {op.shape}, x {x.numel()}, out {out.numel()}")
+        if not (x.is_contiguous() and out.is_contiguous()):
+            raise ValueError("vectors must be contiguous")
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_csr_apply(self._ctx, op.nrows, ctypes.c_void_p(_dptr(op.rowptr)),
+                                         ctypes.c_void_p(_dptr(op.col)), ctypes.c_void_p(_dptr(op.w)),
+                                         ctypes.c_void_p(_dptr(op.oidx)), float(alpha),
+                                         ctypes.c_void_p(_dptr(x)), ctypes.c_void_p(_dptr(out)),
+                                         int(accumulate)))
+        return out
+
+
+class SparseRows:
+    """A real sparse matrix (shape m x n) kept on the device as its non-empty
+    rows: rowptr/col/w in row-compressed form and oidx, the output row of each
+    stored row.  Built from a scipy.sparse matrix."""
+
+    def __init__(self, A, device):
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A, dtype=np.float64)
+        A.sum_duplicates()
+        A.eliminate_zeros()
+        self.shape = A.shape
+        counts = np.diff(A.indptr)
+        rows = np.nonzero(counts)[0]
+        sub = A[rows]
+        self.nrows = int(rows.size)
+        dev = torch.device(device)
+        self.rowptr = torch.from_numpy(sub.indptr.astype(np.int64)).to(dev)
+        self.col = torch.from_numpy(sub.indices.astype(np.int64)).to(dev)
+        self.w = torch.from_numpy(sub.data.astype(np.float64)).to(dev)
+        self.oidx = torch.from_numpy(rows.astype(np.int64)).to(dev)
+        self.host = A
 
 
 # ---------------------------------------------------------------------------

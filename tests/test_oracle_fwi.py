@@ -1,0 +1,155 @@
+"""CPU checks that pin the FWI restatement in oracle/fwi_oracle.py (no GPU).
+
+MATLAB cannot run PDEfunc.m here, so the oracle is pinned by what the
+mathematics fixes:
+  * the gradient of OBJ equals central finite differences of the objective
+    (exact sparse solves) on seeded directions, for s2/m2 and m/s units --
+    a dropped conj, a wrong sign or a forward-instead-of-adjoint stencil fails;
+  * with comp2phys = Ppad' (cut_pml) the gradient is the Pext' gradient
+    restricted to points whose replicas lie inside the physical box;
+  * JACOB_FORW / JACOB_ADJ satisfy Re<J dm, dd> = <dm, J' dd>, and HESS_GN
+    equals J'(J dm);
+  * the sinc/Kaiser interpolation is a unit impulse for points on the grid and
+    reconstructs a band-limited signal between grid points;
+  * Pext' is the transpose of Pext (dot identity) and Pext is edge padding.
+Product host code (helm3d.fwi set-up helpers) is compared with the oracle
+where it has no GPU part.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import fwi_oracle as F
+
+N_PHYS = (9, 8, 7)
+D = (50.0, 50.0, 50.0)
+NPML = np.array([[3, 3, 3], [3, 3, 3]])
+
+
+def _problem(unit="s2/m2", cut_pml=False, freq=(5.0, 6.0)):
+    o = (0.0, 0.0, 0.0)
+    src = (np.array([100.0, 300.0]), np.array([150.0]), np.array([50.0]))
+    rec = (np.array([50.0, 200.0, 350.0]), np.array([100.0, 250.0]), np.array([100.0]))
+    return F.Problem(o, D, N_PHYS, freq, src, rec, NPML, unit=unit, cut_pml=cut_pml)
+
+
+def _models(unit, seed=3):
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(N_PHYS))
+    v_true = 2000.0 + 300.0 * rng.standard_normal(n).clip(-2, 2)
+    v0 = np.full(n, 2000.0)
+    if unit == "m/s":
+        return v_true, v0
+    return 1.0 / v_true ** 2, 1.0 / v0 ** 2
+
+
+def test_assemble_matches_stencil():
+    rng = np.random.default_rng(1)
+    n = (7, 6, 5)
+    N = int(np.prod(n))
+    wn = rng.uniform(1e-4, 1e-3, N)
+    for var in ("forw", "adj", "forw_deriv"):
+        A = F.assemble(var, wn, (10.0, 10.0, 10.0), n, NPML[:, :] * 0 + 2)
+        x = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+        y = O.hmvp(var, wn, (10.0, 10.0, 10.0), n, NPML * 0 + 2, x)
+        assert np.linalg.norm(A @ x - y) <= 1e-14 * np.linalg.norm(y)
+
+
+def test_sinc_on_grid_is_impulse():
+    xout = 25.0 * np.arange(-3, 12)
+    W = F.sinc_weights(np.array([0.0, 100.0, 175.0]), xout)
+    for c, x in enumerate((0.0, 100.0, 175.0)):
+        e = np.zeros(xout.size)
+        e[np.argmin(abs(xout - x))] = 1.0
+        assert np.allclose(W[:, c], e, atol=1e-15)
+
+
+def test_sinc_reconstructs_bandlimited_signal():
+    dx = 10.0
+    xin = dx * np.arange(0, 60)
+    pts = np.array([203.7, 288.1, 311.9])
+    W = F.sinc_weights(pts, xin)           # grid -> points is W'
+    for kk in (0.05, 0.12):
+        s = np.cos(2 * np.pi * kk * xin / dx)
+        rec = W.T @ s
+        assert np.max(abs(rec - np.cos(2 * np.pi * kk * pts / dx))) < 2e-2
+    # the windowed kernel at half-integer offsets (closed form)
+    d = 0.5
+    val = np.i0(6.31 * np.sqrt(1 - (d / 4) ** 2)) / np.i0(6.31) * np.sin(np.pi * d) / (np.pi * d)
+    W1 = F.sinc_weights(np.array([5.0]), dx * np.arange(3))
+    assert abs(W1[0, 0] - val) < 1e-15 and abs(W1[1, 0] - val) < 1e-15
+
+
+def test_product_interp_matches_oracle():
+    from helm3d.fwi import interp_1d
+    xin = np.array([12.0, 37.5, 100.0, 181.25])
+    xout = -75.0 + 25.0 * np.arange(20)
+    W = F.sinc_weights(xin, xout)
+    Wp = interp_1d(xin, xout)
+    assert np.max(abs(W - Wp)) <= 1e-13
+
+
+def test_extension_pair():
+    rng = np.random.default_rng(4)
+    P = F.extension(N_PHYS, [[2, 1, 3], [1, 2, 0]])
+    x = rng.standard_normal(P.shape[1])
+    y = rng.standard_normal(P.shape[0])
+    assert abs(np.dot(P @ x, y) - np.dot(x, P.T @ y)) <= 1e-12 * abs(np.dot(x, P.T @ y))
+    X = x.reshape(N_PHYS[2], N_PHYS[1], N_PHYS[0])
+    ref = np.pad(X, ((3, 0), (1, 2), (2, 1)), mode="edge").reshape(-1)
+    assert np.array_equal(P @ x, ref)
+
+
+@pytest.mark.parametrize("unit", ["s2/m2", "m/s"])
+def test_gradient_matches_finite_differences(unit):
+    P = _problem(unit=unit, cut_pml=False)
+    m_true, m0 = _models(unit)
+    Dobs = P.forward(m_true)
+    f0, g = P.objective(m0, Dobs)
+    rng = np.random.default_rng(7)
+    scale = np.abs(m0).max()
+    for _ in range(3):
+        dm = rng.standard_normal(m0.size) * scale
+        eps = 1e-5      # central differences: error O(eps^2), about 3e-8 relative here
+        fp, _ = P.objective(m0 + eps * dm, Dobs)
+        fm, _ = P.objective(m0 - eps * dm, Dobs)
+        fd = (fp - fm) / (2 * eps)
+        assert abs(fd - np.dot(g, dm)) <= 1e-6 * abs(fd), (fd, np.dot(g, dm))
+    # second-order Taylor remainder: f(m + t dm) - f(m) - t g.dm = O(t^2)
+    dm = rng.standard_normal(m0.size) * scale
+    r = []
+    for t in (1e-3, 5e-4):
+        ft, _ = P.objective(m0 + t * dm, Dobs)
+        r.append(abs(ft - f0 - t * np.dot(g, dm)))
+    assert 3.0 < r[0] / r[1] < 5.0
+
+
+def test_gradient_cut_pml_is_interior_restriction():
+    unit = "s2/m2"
+    m_true, m0 = _models(unit)
+    Pe = _problem(unit=unit, cut_pml=False)
+    Pc = _problem(unit=unit, cut_pml=True)
+    Dobs = Pe.forward(m_true)
+    _, ge = Pe.objective(m0, Dobs)
+    _, gc = Pc.objective(m0, Dobs)
+    G_e = ge.reshape(N_PHYS[::-1])
+    G_c = gc.reshape(N_PHYS[::-1])
+    inner = (slice(1, -1),) * 3
+    assert np.allclose(G_c[inner], G_e[inner], rtol=1e-12, atol=1e-12 * abs(ge).max())
+    # on the faces the truncated gradient misses the PML replicas
+    assert not np.allclose(G_c, G_e, rtol=1e-3)
+
+
+def test_jacobian_adjoint_pair_and_gauss_newton():
+    unit = "s2/m2"
+    P = _problem(unit=unit, cut_pml=False, freq=(5.0,))
+    _, m0 = _models(unit)
+    rng = np.random.default_rng(11)
+    dm = rng.standard_normal(m0.size) * m0.max()
+    Jdm = P.jacob_forw(m0, dm)
+    dd = rng.standard_normal(Jdm.shape) + 1j * rng.standard_normal(Jdm.shape)
+    lhs = np.real(np.vdot(dd, Jdm))
+    rhs = np.dot(dm, P.jacob_adj(m0, dd))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    hgn = P.hess_gn(m0, dm)
+    assert np.linalg.norm(hgn - P.jacob_adj(m0, Jdm)) <= 1e-10 * np.linalg.norm(hgn)
