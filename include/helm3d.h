@@ -198,6 +198,37 @@ int helm3d_solve_mixed(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev
 int helm3d_solve_bicgstab(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
                           const helm3d_solve_opts *opts, helm3d_solve_info *info);
 
+/* ---- batched solves --------------------------------------------------------- */
+
+/* Many right-hand sides, possibly on different operators and grids, solved
+ * together: every kernel type runs as one launch over all items per Krylov
+ * step (the stencil from a device work list of (operator, vector) items), the
+ * host synchronises once per step for the whole batch, and the complex64
+ * inner iterations of the mixed method are replayed from one captured HIP
+ * graph covering all items.  No counterpart in the MEX layer; the plain
+ * method follows FGMRES_MT.m:40-135 (the multi-RHS branch of FGMRES.m:35-37):
+ * lockstep cycles, per-item Hessenberg least squares, the step loop runs
+ * while the largest quasi-residual is above tol and the cycle loop while the
+ * largest true residual is. */
+typedef struct helm3d_batch helm3d_batch;
+
+#define HELM3D_BATCH_PLAIN 0  /* complex128 GMRES(maxinnerit), FGMRES_MT semantics          */
+#define HELM3D_BATCH_MIXED 1  /* FP64 flexible GMRES + inner_maxit complex64 BiCGStab        */
+                              /* iterations per outer step; items stop individually        */
+
+int helm3d_batch_create(int device, void *stream, helm3d_batch **out);
+int helm3d_batch_destroy(helm3d_batch *batch);
+int helm3d_batch_set_stream(helm3d_batch *batch, void *stream);
+/* Item i solves A_i x_i = b_i with A_i = H(ctx[i]) (mode 1) or H(ctx[i])^H
+ * (otherwise); b[i], x[i]: device complex128 vectors of ctx[i]'s size; items
+ * may share a context.  Every context must hold a model (all with real wn,
+ * or all with complex wn) on the batch's device.  x starts from 0
+ * (opts->use_x0 must be 0).  info: n entries (may be NULL).  Returns
+ * HELM3D_ERR_NOTCONV when some item ended above tol. */
+int helm3d_batch_solve(helm3d_batch *batch, int n, helm3d_ctx *const *ctx, int mode,
+                       const void *const *b, void *const *x, int method,
+                       const helm3d_solve_opts *opts, helm3d_solve_info *info);
+
 /* ---- FWI gradient ------------------------------------------------------- */
 
 #define HELM3D_GRID_PHYS 0  /* g on the physical grid: comp2phys = truncation  */

@@ -12,85 +12,24 @@
 //
 // Scalar slots (doubles, device): see the BS_* offsets in bicgstab.hpp.
 #include "bicgstab.hpp"
+#include "kbody.hpp"
 #include "krylov.hpp"
 
 namespace helm3d {
 
 namespace {
 
-constexpr int KB = 256;
-
-__device__ inline double wave_sum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double* red)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) red[wid * NV + q] = v[q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double s = red[q];
-            for (int w = 1; w < KB / 64; ++w) s += red[w * NV + q];
-            v[q] = s;
-        }
-    }
-}
-
-__device__ inline void block_range(long n, long& lo, long& hi)
-{
-    const long per = (n + gridDim.x - 1) / gridDim.x;
-    lo = long(blockIdx.x) * per;
-    hi = min(n, lo + per);
-}
-
-struct C {
-    double re, im;
-};
-__device__ inline C cdiv(C a, C b)
-{
-    const double d = b.re * b.re + b.im * b.im;
-    if (d == 0.0) return C{0.0, 0.0};
-    return C{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
-}
-__device__ inline C cmulc(C a, C b) { return C{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ inline C ld(const double* s, int off) { return C{s[off], s[off + 1]}; }
+using kb::KB;
+using kb::C;
+using kb::cdiv;
+using kb::cmulc;
+using kb::ld;
 
 // Partial sums of conj(a_q) . b_q for NP pairs; optionally ||a_last||^2.
 template <typename R, int NP, bool NORM>
 __global__ __launch_bounds__(KB) void dots_pairs_kernel(PairArgs<R> P, long n, double* __restrict__ part)
 {
-    constexpr int NV = 2 * NP + (NORM ? 1 : 0);
-    __shared__ double red[(KB / 64) * NV];
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    long lo, hi;
-    block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-            const cpx<R> a = P.a[q][t], b = P.b[q][t];
-            acc[2 * q] += double(a.re) * double(b.re) + double(a.im) * double(b.im);
-            acc[2 * q + 1] += double(a.re) * double(b.im) - double(a.im) * double(b.re);
-            if (NORM && q == NP - 1) acc[NV - 1] += double(a.re) * double(a.re) + double(a.im) * double(a.im);
-        }
-    }
-    block_sum<NV>(acc, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) part[long(q) * gridDim.x + blockIdx.x] = acc[q];
-    }
+    kb::dots_pairs_body<R, NP, NORM>(P, n, part, blockIdx.x, gridDim.x);
 }
 
 // s = r - a v with a = rho / (rhat, v); block 0 records a.
@@ -120,37 +59,7 @@ __global__ __launch_bounds__(KB) void bicg_xr_kernel(const cpx<R>* __restrict__ 
                                                      cpx<R>* __restrict__ z, cpx<R>* __restrict__ r, long n,
                                                      double* __restrict__ sc, double* __restrict__ part)
 {
-    __shared__ double red[(KB / 64) * 3];
-    const C a = ld(sc, BS_ALPHA);
-    const double tn = sc[BS_TT];
-    const C w = tn > 0.0 ? C{sc[BS_TS] / tn, sc[BS_TS + 1] / tn} : C{0.0, 0.0};
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        sc[BS_OMEGA] = w.re;
-        sc[BS_OMEGA + 1] = w.im;
-    }
-    double acc[3] = {0.0, 0.0, 0.0};
-    long lo, hi;
-    block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
-        const cpx<R> pv = p[t], sv = s[t], tv = tt[t], hv = rhat[t], zv = z[t];
-        double zr = double(zv.re) + a.re * double(pv.re) - a.im * double(pv.im) + w.re * double(sv.re) -
-                    w.im * double(sv.im);
-        double zi = double(zv.im) + a.re * double(pv.im) + a.im * double(pv.re) + w.re * double(sv.im) +
-                    w.im * double(sv.re);
-        z[t] = cpx<R>{R(zr), R(zi)};
-        const double rr = double(sv.re) - (w.re * double(tv.re) - w.im * double(tv.im));
-        const double ri = double(sv.im) - (w.re * double(tv.im) + w.im * double(tv.re));
-        const cpx<R> rn{R(rr), R(ri)};
-        r[t] = rn;
-        acc[0] += double(hv.re) * double(rn.re) + double(hv.im) * double(rn.im);
-        acc[1] += double(hv.re) * double(rn.im) - double(hv.im) * double(rn.re);
-        acc[2] += double(rn.re) * double(rn.re) + double(rn.im) * double(rn.im);
-    }
-    block_sum<3>(acc, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) part[long(q) * gridDim.x + blockIdx.x] = acc[q];
-    }
+    kb::bicg_xr_body<R>(p, s, tt, rhat, z, r, n, sc, part, blockIdx.x, gridDim.x);
 }
 
 // p = r + beta (p - w v), beta = (rho_new / rho_old)(a / w).

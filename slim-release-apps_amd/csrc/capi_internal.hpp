@@ -1,0 +1,201 @@
+// Internal host-side declarations shared by the C-ABI sources: error
+// reporting, device allocation helpers, the Hessenberg least squares and the
+// operator context.
+#pragma once
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bicgstab.hpp"
+#include "fwi.hpp"
+#include "helm27.hpp"
+#include "helm3d.h"
+#include "krylov.hpp"
+
+namespace helm3d {
+
+// message of the calling thread's last failure (defined in helm3d_capi.cpp)
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(e_ == hipErrorOutOfMemory ? HELM3D_ERR_NOMEM : HELM3D_ERR_HIP,              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+#define CHK(expr)                      \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != HELM3D_OK) return s_; \
+    } while (0)
+
+// Grow-only device allocation.
+inline int ensure(void** p, size_t* cap, size_t bytes)
+{
+    if (*cap >= bytes && *p) return HELM3D_OK;
+    if (*p) {
+        HIPCHK(hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    if (bytes == 0) return HELM3D_OK;
+    HIPCHK(hipMalloc(p, bytes));
+    *cap = bytes;
+    return HELM3D_OK;
+}
+
+template <typename T>
+inline int ensure_t(T** p, size_t* cap, size_t count)
+{
+    void* q = *p;
+    int s = ensure(&q, cap, count * sizeof(T));
+    *p = static_cast<T*>(q);
+    return s;
+}
+
+// Least squares min ||beta e1 - H y|| for the leading (k+1) x k block of the
+// column-major (ldh x *) Hessenberg matrix H, by Givens rotations; returns the
+// explicit residual ||beta e1 - H y|| as FGMRES.m:109-111 forms it.
+inline double hess_lsq(int k, int ldh, const std::vector<std::complex<double>>& H, double beta,
+                std::vector<std::complex<double>>& y)
+{
+    using C = std::complex<double>;
+    std::vector<C> R((k + 1) * k), g(k + 1, C(0));
+    for (int c = 0; c < k; ++c)
+        for (int r = 0; r <= k; ++r) R[r + c * (k + 1)] = (r <= c + 1) ? H[r + c * ldh] : C(0);
+    g[0] = beta;
+    for (int c = 0; c < k; ++c) {
+        const C a = R[c + c * (k + 1)], b = R[c + 1 + c * (k + 1)];
+        const double den = std::sqrt(std::norm(a) + std::norm(b));
+        C cs(1), sn(0);
+        if (den != 0) {
+            cs = a / den;
+            sn = b / den;
+        }
+        for (int cc = c; cc < k; ++cc) {
+            const C u = R[c + cc * (k + 1)], v = R[c + 1 + cc * (k + 1)];
+            R[c + cc * (k + 1)] = std::conj(cs) * u + std::conj(sn) * v;
+            R[c + 1 + cc * (k + 1)] = -sn * u + cs * v;
+        }
+        const C u = g[c], v = g[c + 1];
+        g[c] = std::conj(cs) * u + std::conj(sn) * v;
+        g[c + 1] = -sn * u + cs * v;
+    }
+    y.assign(k, C(0));
+    for (int r = k - 1; r >= 0; --r) {
+        C s = g[r];
+        for (int c = r + 1; c < k; ++c) s -= R[r + c * (k + 1)] * y[c];
+        y[r] = s / R[r + r * (k + 1)];
+    }
+    double rs = 0;
+    for (int r = 0; r <= k; ++r) {
+        C s = (r == 0) ? C(beta) : C(0);
+        for (int c = 0; c < k; ++c)
+            if (r <= c + 1) s -= H[r + c * ldh] * y[c];
+        rs += std::norm(s);
+    }
+    return std::sqrt(rs);
+}
+
+template <typename R>
+struct Vecs {
+    cpx<R>* p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace helm3d
+
+using namespace helm3d;
+
+struct helm3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    bool has_model = false;
+    int n[3] = {0, 0, 0};
+    int npml[6] = {0, 0, 0, 0, 0, 0};
+    double h[3] = {0, 0, 0};
+    bool wnc = false;
+    long N = 0;
+    HConst hc{};
+
+    cpx<double>* tab64 = nullptr;
+    size_t tab64_cap = 0;
+    cpx<float>* tab32 = nullptr;
+    size_t tab32_cap = 0;
+    void* wn64 = nullptr;
+    size_t wn64_cap = 0;
+    void* wn32 = nullptr;
+    size_t wn32_cap = 0;
+    OpDesc<double> op64{};
+    OpDesc<float> op32{};
+    StencilGrid grid{};
+
+    double* part = nullptr;
+    size_t part_cap = 0;
+    void* sink = nullptr;  // stencil sink (kSinkSlots complex doubles)
+    size_t sink_cap = 0;
+    double* small = nullptr;  // device: 2*kMaxBasis dots + norm
+    size_t small_cap = 0;
+    double* small_host = nullptr;  // pinned mirror
+
+    Vecs<double> V64, Z64, T64, BC64;
+    Vecs<float> V32, T32a, T32b, BC32;
+    double* bsc = nullptr;  // BiCGStab device scalars (BS_SLOTS doubles)
+    size_t bsc_cap = 0;
+
+    // captured fixed-length complex64 inner solve (mixed precision)
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t inner_graph = nullptr;
+    int inner_graph_variant = -1;
+    long inner_graph_iters = -1;
+    bool use_graphs = true;
+
+    // optional per-kernel-class timing with HIP events on `stream`
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_pending;  // (kind, index of start event)
+    double prof_ms[HELM3D_PROF_KINDS] = {0};
+    long long prof_n[HELM3D_PROF_KINDS] = {0};
+
+    ~helm3d_ctx()
+    {
+        (void)hipSetDevice(device);
+        void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, BC64.p,
+                        V32.p, T32a.p, T32b.p, BC32.p, bsc};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (small_host) (void)hipHostFree(small_host);
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        if (inner_graph) (void)hipGraphExecDestroy(inner_graph);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    }
+};
+
+namespace helm3d {
+
+inline int bind(helm3d_ctx* c)
+{
+    if (!c) return fail(HELM3D_ERR_ARG, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    return HELM3D_OK;
+}
+
+// Leading dimension of a vector slot: N rounded up to 256 complex values so
+// every slot starts on a 4 KiB boundary.
+inline long slot_ld(long N) { return (N + 255) / 256 * 256; }
+
+}  // namespace helm3d

@@ -29,6 +29,15 @@ struct StencilLaunch {
     // write-only sink of kSinkSlots values for steps/lanes with no output
     cpx<R>* sink;
     int tiles_x, tiles_y, kchunk;
+    // fused BiCGStab forms (EPI 3: f0 = r, f1 = p, f2 = v, o2 = new p,
+    // rh = rhat; EPI 4: f0 = r, f1 = v, o2 = s); scalars in sc
+    const cpx<R>* f0;
+    const cpx<R>* f1;
+    const cpx<R>* f2;
+    cpx<R>* o2;
+    const cpx<R>* rh;
+    double* sc;
+    int rho_new, rho_old, first;
 };
 
 // Launch shape: 64 x ty column tiles, z chunks of kchunk planes, pf planes of
@@ -45,5 +54,14 @@ StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty = 4, int pf 
 
 template <typename R>
 hipError_t launch_stencil(int variant, StencilLaunch<R> L, const StencilGrid& g, hipStream_t s);
+
+// One launch over nitems device-resident StencilLaunch items (operators
+// without DERIV); item it owns blocks [first[it], first[it+1]) of `blocks`,
+// launched with its own tiles_x/tiles_y/kchunk and the common shape (ty, pf).
+// epi: 0 (y = alpha H x), 1 (+ beta b), 3, 4 (fused BiCGStab forms).
+// active (nullable): items with active[it] == 0 do nothing.
+template <typename R>
+hipError_t launch_stencil_batch(int variant, int epi, const StencilLaunch<R>* items, const int* first, int nitems,
+                                int blocks, const int* active, int ty, int pf, hipStream_t s);
 
 }  // namespace helm3d

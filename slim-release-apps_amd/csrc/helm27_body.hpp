@@ -1,0 +1,494 @@
+// Device body of the 27-point stencil (included by the kernel files).
+//
+// and complex-wavenumber forms in one z-marching kernel template.
+//
+// Reference semantics: do_Hmvp of SLIM-release-apps
+// tools/algorithms/3DFreqModeling/Helm3d_27pt.h:827-875 with the coefficients
+// of :680-714 (forward) and :718-822 (adjoint), the DERIV / WNCMPLX variants of
+// :4-16, 202-206, and zero padding outside the grid (:263-317).
+//
+// Formulation.  Every coefficient is linear in the PML values of one point, so
+// the row of H at p = (i,j,k) splits into
+//   * per-column complex coefficient sets F (in-plane neighbours) and G
+//     (neighbours in planes k+-1) that carry the x/y PML of the column and are
+//     fixed for the whole z-march,
+//   * a real 9-point in-plane stencil Qz that carries the z PML, weighted per
+//     plane by xi_z^lo, xi_z^hi, xi_z (uniform across the block),
+//   * the mass-lumping stencil (wn_coef, -WM2, -WM3, -WM4 by |d|_1) acting on
+//     z = wn .* x (forward) or on x with conj(wn) at the centre (adjoint).
+// The adjoint is the exact conjugate transpose: F/G take the neighbour's x/y
+// PML values and the z weights come from the neighbouring planes.
+//
+// Tiling.  A 256-thread block owns a 64 x 4 column tile and marches over a z
+// chunk.  Each plane (with a one-point x/y halo) is staged once in LDS
+// (double-buffered, one barrier per plane); the next plane's global loads are
+// issued before the barrier so HBM latency overlaps the current plane.  Block
+// ids are remapped so consecutive tiles of the same chunk land on the same XCD
+// and share halo lines in its L2.
+#pragma once
+
+#include <type_traits>
+
+#include "bicgstab.hpp"
+#include "helm27.hpp"
+
+namespace helm3d {
+
+namespace {
+
+constexpr int TX = 64;  // one wavefront along x
+constexpr int HX = TX + 2;
+constexpr int kMaxChunk = 254;  // z planes per block (LDS holds their PML weights)
+
+template <typename R>
+__device__ inline cpx<R> zero() { return cpx<R>{R(0), R(0)}; }
+
+// Per-column coefficient sets.  For neighbour offset (dx,dy) (a = dx+1,
+// b = dy+1) with side values xs[a], ys[b] (xs[1], ys[1] = full xi):
+//   F (dz = 0):  centre -(c_x xi_x + c_y xi_y); x faces c_x xi_x^s + w_xy xi_y;
+//                y faces c_y xi_y^s + w_xy xi_x; diagonals -w_xy (xi_x^s + xi_y^s)
+//   G (dz=+-1):  centre w_xz xi_x + w_yz xi_y; x faces -w_xz xi_x^s + w3a2 xi_y;
+//                y faces -w_yz xi_y^s + w3a2 xi_x; diagonals -w3a2 (xi_x^s + xi_y^s)
+// i.e. the non-xi_z part of the coefficients of Helm3d_27pt.h:680-714.
+template <typename R>
+__device__ inline void column_coefs(const OpDesc<R>& op, const cpx<R> xs[3], const cpx<R> ys[3],
+                                    cpx<R> F[3][3], cpx<R> G[3][3])
+{
+    const cpx<R> xf = xs[1], yf = ys[1];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (a == 1 && b == 1) {
+                F[a][b] = -(op.c[0] * xf + op.c[1] * yf);
+                G[a][b] = op.xz * xf + op.yz * yf;
+            } else if (b == 1) {
+                F[a][b] = op.c[0] * xs[a] + op.xy * yf;
+                G[a][b] = (-op.xz) * xs[a] + op.w3a2 * yf;
+            } else if (a == 1) {
+                F[a][b] = op.c[1] * ys[b] + op.xy * xf;
+                G[a][b] = (-op.yz) * ys[b] + op.w3a2 * xf;
+            } else {
+                F[a][b] = (-op.xy) * (xs[a] + ys[b]);
+                G[a][b] = (-op.w3a2) * (xs[a] + ys[b]);
+            }
+        }
+    }
+}
+
+// FP64 complex scalars of the fused BiCGStab forms (device slots, see
+// bicgstab.hpp); a zero denominator gives zero (the breakdown guard of the
+// BiCGStab kernels).
+struct Cd {
+    double re, im;
+};
+__device__ inline Cd cd_ld(const double* s, int off) { return Cd{s[off], s[off + 1]}; }
+__device__ inline Cd cd_mul(Cd a, Cd b) { return Cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline Cd cd_div(Cd a, Cd b)
+{
+    const double d = b.re * b.re + b.im * b.im;
+    if (d == 0.0) return Cd{0.0, 0.0};
+    return Cd{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+// TY: rows of the column tile (block = 64 x TY threads); PF: planes of global
+// loads kept in flight ahead of the plane being computed (1 or 2).
+//
+// Memory pipeline: every global load of the loop goes into one of PF register
+// sets that are refilled right after being staged into LDS, so no register is
+// ever copied while its load is in flight and the only vmcnt waits are those
+// of the set being staged (PF planes after it was requested).  The per-plane z
+// PML weights are staged in LDS once per block, and the own-point operands of
+// the epilogue (adjoint wn, addend b, gradient U and g) travel in the same set
+// as the halo slots of the next plane.
+// EPI: 0 stores y = alpha*H x, 1 stores y = alpha*H x + beta*b, 2 runs the
+// gradient epilogue (accumulates Re(conj(U) . out) into g).  3 and 4 are the
+// two operator applications of a BiCGStab iteration with the vector updates
+// around them fused in (scalars read from the device slots L.sc):
+//   3: x = r + beta (p - omega v) formed while staging, p <- x stored at the
+//      own point, y = v = H x, partial (rhat, v)
+//   4: x = s = r - alpha v formed while staging, s stored, y = t = H s,
+//      partials (t, s) and ||t||^2
+// The body runs one block `lin` of a launch of `G` blocks described by L.
+template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI>
+__device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int lin, const int G)
+{
+    constexpr int NT = TX * TY;
+    constexpr int HY = TY + 2;
+    constexpr int NS = HX * HY;
+    static_assert(NS <= 2 * NT, "halo tile must fit two load slots per thread");
+    static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
+    const OpDesc<R>& op = L.op;
+    __shared__ cpx<R> su[2][HY][HX];            // input vector plane (+halo)
+    __shared__ cpx<R> sm[ADJ ? 1 : 2][HY][HX];  // forward: wn .* x plane (+halo)
+    __shared__ cpx<R> szl[kMaxChunk + 4], szh[kMaxChunk + 4];  // z PML slots k0 .. k1+3
+
+    const int nx = op.nx, ny = op.ny, nz = op.nz;
+
+    // XCD-aware block remap: the first 8*floor(G/8) blocks are dealt so that
+    // each XCD (round-robin by block id) receives a contiguous logical range.
+    const int G8 = G & ~7;
+    const int logical = lin < G8 ? (lin & 7) * (G8 >> 3) + (lin >> 3) : lin;
+    const int ntile = L.tiles_x * L.tiles_y;
+    const int chunk = logical / ntile;
+    const int tile = logical - chunk * ntile;
+    const int tyt = tile / L.tiles_x;
+    const int i0 = (tile - tyt * L.tiles_x) * TX;
+    const int j0 = tyt * TY;
+    const int k0 = chunk * L.kchunk;
+    const int k1 = min(nz, k0 + L.kchunk);
+
+    const int tid = threadIdx.x;
+    const int tx = tid & (TX - 1), ty = tid / TX;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = (i < nx) && (j < ny);
+    const int ic = min(i, nx - 1), jc = min(j, ny - 1);  // clamped for table reads
+    // A wave is one row j of 64 columns: it takes the interior path when every
+    // column and the row have xi_lo = xi_hi = 1 at index-1, index, index+1.
+    const int jw = __builtin_amdgcn_readfirstlane(j);
+    const bool interior = i0 >= op.int_lo[0] && i0 + TX - 1 <= op.int_hi[0] && jw >= op.int_lo[1] &&
+                          jw <= op.int_hi[1];
+
+    // ---- per-column coefficients (fixed for the whole march)
+    cpx<R> F[3][3], Gc[3][3];
+    if constexpr (!DERIV) {
+        cpx<R> xs[3], ys[3];
+        if constexpr (!ADJ) {
+            xs[0] = op.xlo[ic + 1]; xs[2] = op.xhi[ic + 1]; xs[1] = xs[0] + xs[2];
+            ys[0] = op.ylo[jc + 1]; ys[2] = op.yhi[jc + 1]; ys[1] = ys[0] + ys[2];
+        } else {
+            // neighbour i-1 couples through its hi value, i+1 through its lo value
+            xs[0] = op.xhi[ic]; xs[2] = op.xlo[ic + 2]; xs[1] = op.xlo[ic + 1] + op.xhi[ic + 1];
+            ys[0] = op.yhi[jc]; ys[2] = op.ylo[jc + 2]; ys[1] = op.ylo[jc + 1] + op.yhi[jc + 1];
+        }
+        column_coefs(op, xs, ys, F, Gc);
+        if constexpr (ADJ) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) { F[a][b] = conjc(F[a][b]); Gc[a][b] = conjc(Gc[a][b]); }
+        }
+        // z weights of this chunk: slots k0 .. k1+1 (planes k0-1 .. k1) plus
+        // two padding entries read by the trailing pipeline step
+        for (int q = tid; q < k1 - k0 + 4; q += NT) {
+            const int sl = min(k0 + q, nz + 1);
+            szl[q] = op.zlo[sl];
+            szh[q] = op.zhi[sl];
+        }
+    }
+
+    // ---- load slots: the (TX+2) x (TY+2) halo tile, two slots per thread
+    const long plane = long(nx) * ny;
+    const int s0 = tid, s1 = tid + NT;
+    const int s0x = s0 % HX, s0y = s0 / HX;
+    const int s1x = s1 % HX, s1y = s1 / HX;
+    const bool has1 = s1 < NS;
+    const int g0i = i0 - 1 + s0x, g0j = j0 - 1 + s0y;
+    const int g1i = i0 - 1 + s1x, g1j = j0 - 1 + s1y;
+    const bool in0 = g0i >= 0 && g0i < nx && g0j >= 0 && g0j < ny;
+    const bool in1 = has1 && g1i >= 0 && g1i < nx && g1j >= 0 && g1j < ny;
+    const long off0 = in0 ? g0i + long(g0j) * nx : 0;
+    const long off1 = in1 ? g1i + long(g1j) * nx : 0;
+    const long offc = long(ic) + long(jc) * nx;
+    // gradient epilogue: position of the own column in the physical box
+    const int qi = i - L.p0[0], qj = j - L.p0[1];
+    const bool gcol = EPI == 2 && active && qi >= 0 && qi < L.pn[0] && qj >= 0 && qj < L.pn[1];
+    const long gcol_off = gcol ? qi + long(qj) * L.pn[0] : 0;
+    const long gplane = long(L.pn[0]) * L.pn[1];
+    // real-wn loads (forward); with a constant wn they read a valid dummy address
+    const R* wnp = op.wn ? op.wn : reinterpret_cast<const R*>((EPI == 3 || EPI == 4) ? L.f0 : L.x);
+
+    // block partial sums: ||y||^2 (EPI 0/1), (rhat, v) (EPI 3), (t, s) and
+    // ||t||^2 (EPI 4)
+    constexpr int NV = EPI == 3 ? 2 : (EPI == 4 ? 3 : 1);
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    const int sink_slot = int((long(lin) * NT + tid) % kSinkSlots);
+    const int sink_slot2 = (sink_slot + kSinkSlots / 2) % kSinkSlots;
+
+    // BiCGStab coefficients of the staged combination x = r + ca*f1 + cb*f2
+    // (EPI 3: ca = beta, cb = -beta*omega; EPI 4: ca = -alpha, cb = 0)
+    cpx<R> ca = zero<R>(), cb = zero<R>();
+    if constexpr (EPI == 3 || EPI == 4) {
+        const double* sc = L.sc;
+        if constexpr (EPI == 3) {
+            if (!L.first) {
+                const Cd beta = cd_mul(cd_div(cd_ld(sc, L.rho_new), cd_ld(sc, L.rho_old)),
+                                       cd_div(cd_ld(sc, BS_ALPHA), cd_ld(sc, BS_OMEGA)));
+                const Cd bo = cd_mul(beta, cd_ld(sc, BS_OMEGA));
+                ca = cpx<R>{R(beta.re), R(beta.im)};
+                cb = cpx<R>{R(-bo.re), R(-bo.im)};
+            }
+        } else {
+            const Cd a = cd_div(cd_ld(sc, L.rho_new), cd_ld(sc, BS_RV));
+            if (lin == 0 && tid == 0) {
+                L.sc[BS_ALPHA] = a.re;
+                L.sc[BS_ALPHA + 1] = a.im;
+            }
+            ca = cpx<R>{R(-a.re), R(-a.im)};
+        }
+    }
+    cpx<R> Xlast = zero<R>();  // EPI 3/4: staged x at the own point of the previous plane
+
+    // One register set: halo slots of plane q and the own-point operands of
+    // plane q-1 (the plane output when q is staged).  Every load is issued
+    // unconditionally from a clamped, in-bounds address and validity is
+    // applied when the set is consumed, so the number of loads in flight is
+    // fixed and the compiler waits only for the set it stages.
+    struct Set {
+        cpx<R> x0, x1;  // slots (EPI 3/4: r)
+        cpx<R> e0, e1;  // EPI 3: p, EPI 4: v at the slots
+        cpx<R> f0, f1;  // EPI 3: v at the slots
+        R w0, w1;       // forward, real wn at the slots
+        cpx<R> c0, c1;  // forward, complex wn at the slots
+        cpx<R> wc;      // adjoint: wn at own point of plane q-1
+        cpx<R> bb;      // addend b at own point of plane q-1
+        cpx<R> gu;      // gradient: U at own point of plane q-1
+        double gv;      // gradient: g at own point of plane q-1
+    };
+    auto fetch = [&](int q, Set& S) {
+        const long base = long(min(max(q, 0), nz - 1)) * plane;
+        if constexpr (EPI == 3 || EPI == 4) {
+            S.x0 = L.f0[base + off0];
+            S.x1 = L.f0[base + off1];
+            S.e0 = L.f1[base + off0];
+            S.e1 = L.f1[base + off1];
+            if constexpr (EPI == 3) {
+                S.f0 = L.f2[base + off0];
+                S.f1 = L.f2[base + off1];
+            }
+        } else {
+            S.x0 = L.x[base + off0];
+            S.x1 = L.x[base + off1];
+        }
+        if constexpr (!ADJ) {
+            if constexpr (WNC) {
+                S.c0 = op.wnc[base + off0];
+                S.c1 = op.wnc[base + off1];
+            } else {
+                S.w0 = wnp[base + off0];
+                S.w1 = wnp[base + off1];
+            }
+        }
+        const long own = long(min(max(q - 1, 0), nz - 1)) * plane + offc;
+        if constexpr (ADJ) {
+            if constexpr (WNC) S.wc = op.wnc[own];
+            else S.wc = cpx<R>{wnp[own], R(0)};
+        }
+        if constexpr (EPI == 1) S.bb = L.b[own];
+        if constexpr (EPI == 3) S.bb = L.rh[own];
+        if constexpr (EPI == 2) {
+            S.gu = L.gu[own];
+            const int qk = min(max(q - 1 - L.p0[2], 0), L.pn[2] - 1);
+            S.gv = L.gg[gcol_off + long(qk) * gplane];
+        }
+    };
+
+    // pipeline state (see the header comment of this file)
+    cpx<R> Qprev = zero<R>(), Qcur = zero<R>(), Acc = zero<R>(), Pend = zero<R>();
+    cpx<R> MAcc = zero<R>(), MOcur = zero<R>();
+
+    // Stage set S (plane q) into LDS, refill S with plane q+PF, compute plane
+    // q and emit plane q-1.
+    auto step = [&](int q, int t, Set& S) {
+        const int buf = t & 1;
+        const bool inz = q >= 0 && q < nz;
+        const bool v0 = inz && in0, v1 = inz && in1;
+        cpx<R> x0 = S.x0, x1 = S.x1;
+        if constexpr (EPI == 3 || EPI == 4) {
+            x0 = cfma(x0, ca, S.e0);
+            x1 = cfma(x1, ca, S.e1);
+            if constexpr (EPI == 3) {
+                x0 = cfma(x0, cb, S.f0);
+                x1 = cfma(x1, cb, S.f1);
+            }
+        }
+        x0 = v0 ? x0 : zero<R>();
+        x1 = v1 ? x1 : zero<R>();
+        su[buf][s0y][s0x] = x0;
+        if (has1) su[buf][s1y][s1x] = x1;
+        if constexpr (!ADJ) {
+            if constexpr (WNC) {
+                sm[buf][s0y][s0x] = cmul(S.c0, x0);
+                if (has1) sm[buf][s1y][s1x] = cmul(S.c1, x1);
+            } else {
+                const R w0 = op.wn ? S.w0 : op.wn_const, w1 = op.wn ? S.w1 : op.wn_const;
+                sm[buf][s0y][s0x] = w0 * x0;
+                if (has1) sm[buf][s1y][s1x] = w1 * x1;
+            }
+        }
+        // own-point operands of plane q-1 leave the set before the refill
+        cpx<R> wc = S.wc;
+        if constexpr (ADJ && !WNC) wc.re = op.wn ? wc.re : op.wn_const;
+        const cpx<R> bb = S.bb, gu = S.gu;
+        const double gv = S.gv;
+        fetch(q + PF, S);  // unconditional: a fixed number of loads per step
+        __syncthreads();
+
+        // ---- mass-lumping sums (forward: on wn .* x, adjoint: on x)
+        cpx<R> Mc, Mo;
+        {
+            cpx<R> m[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) m[a][b] = ADJ ? su[buf][ty + b][tx + a] : sm[buf][ty + b][tx + a];
+            const cpx<R> mf = (m[0][1] + m[2][1]) + (m[1][0] + m[1][2]);
+            const cpx<R> md = (m[0][0] + m[2][0]) + (m[0][2] + m[2][2]);
+            Mc = op.wn_coef * m[1][1];
+            Mc = rfma(Mc, -op.wm2, mf);
+            Mc = rfma(Mc, -op.wm3, md);
+            Mo = (-op.wm2) * m[1][1];
+            Mo = rfma(Mo, -op.wm3, mf);
+            Mo = rfma(Mo, -op.wm4, md);
+        }
+
+        // ---- stiffness: Qz (z PML, applied per plane) and F, G (x/y PML)
+        cpx<R> Qn = zero<R>(), Fn = zero<R>(), Gn = zero<R>();
+        if constexpr (!DERIV) {
+            cpx<R> u[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) u[a][b] = su[buf][ty + b][tx + a];
+            // pair sums along y (per column a) and along x (per row b)
+            const cpx<R> sy0 = u[0][0] + u[0][2], sy1 = u[1][0] + u[1][2], sy2 = u[2][0] + u[2][2];
+            const cpx<R> sx1 = u[0][1] + u[2][1];
+            const cpx<R> dg = sy0 + sy2;
+            Qn = op.c[2] * u[1][1];
+            Qn = rfma(Qn, -op.xz, sx1);
+            Qn = rfma(Qn, -op.yz, sy1);
+            Qn = rfma(Qn, -op.w3a2, dg);
+            if (interior) {
+                // x/y PML absent around this wave: F, G are real constant
+                // stencils on the sums already formed for Qz
+                Fn = op.fi[0] * u[1][1];
+                Fn = rfma(Fn, op.fi[1], sx1);
+                Fn = rfma(Fn, op.fi[2], sy1);
+                Fn = rfma(Fn, op.fi[3], dg);
+                Gn = op.gi[0] * u[1][1];
+                Gn = rfma(Gn, op.gi[1], sx1);
+                Gn = rfma(Gn, op.gi[2], sy1);
+                Gn = rfma(Gn, op.gi[3], dg);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        Fn = cfma(Fn, F[a][b], u[a][b]);
+                        Gn = cfma(Gn, Gc[a][b], u[a][b]);
+                    }
+            }
+        }
+
+        cpx<R> out = zero<R>();
+        const bool emit = q >= k0 + 1 && q <= k1;
+        const int kc = q - 1;
+        if constexpr (!ADJ) {
+            // y(kc) = wlo Q(kc-1) + whi Q(kc+1) - (wlo+whi) Q(kc) + GM(kc-1) + GM(kc+1) + FM(kc)
+            const cpx<R> GMn = Gn + Mo;
+            if (emit) {
+                out = Acc + GMn;
+                if constexpr (!DERIV) {
+                    const cpx<R> wl = szl[kc + 1 - k0], wh = szh[kc + 1 - k0];
+                    out = cfma(out, wl, Qprev - Qcur);
+                    out = cfma(out, wh, Qn - Qcur);
+                }
+            }
+            Qprev = Qcur;
+            Qcur = Qn;
+            Acc = Pend + (Fn + Mc);
+            Pend = GMn;
+        } else {
+            // contributions of plane q to outputs q+1 (Sdn), q-1 (Sup), q (S0)
+            cpx<R> Sdn = Gn, Sup = Gn, S0 = Fn;
+            if constexpr (!DERIV) {
+                const cpx<R> zl = szl[q + 1 - k0], zh = szh[q + 1 - k0];
+                Sdn = cfma(Sdn, conjc(zh), Qn);
+                Sup = cfma(Sup, conjc(zl), Qn);
+                S0 = cfma(S0, -conjc(zl + zh), Qn);
+            }
+            if (emit) {
+                out = Acc + Sup;
+                out = cfma(out, conjc(wc), MAcc + Mo);
+            }
+            Acc = Pend + S0;
+            MAcc = MOcur + Mc;
+            Pend = Sdn;
+            MOcur = Mo;
+        }
+        // Exactly one store per step and lane (two for EPI 3/4): steps or
+        // lanes without an output write to the sink instead (keeps the vmcnt
+        // bookkeeping exact).
+        const bool live = emit && active;
+        if constexpr (EPI == 3 || EPI == 4) {
+            const cpx<R> xo = su[buf][ty + 1][tx + 1];  // own point of plane q
+            cpx<R>* dy = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
+            cpx<R>* d2 = live ? &L.o2[long(kc) * plane + offc] : &L.sink[sink_slot2];
+            *dy = out;
+            *d2 = Xlast;
+            if (live) {
+                if constexpr (EPI == 3) {
+                    // (rhat, v) += conj(rhat) v
+                    acc[0] += double(bb.re) * double(out.re) + double(bb.im) * double(out.im);
+                    acc[1] += double(bb.re) * double(out.im) - double(bb.im) * double(out.re);
+                } else {
+                    // (t, s) += conj(t) s ; ||t||^2
+                    acc[0] += double(out.re) * double(Xlast.re) + double(out.im) * double(Xlast.im);
+                    acc[1] += double(out.re) * double(Xlast.im) - double(out.im) * double(Xlast.re);
+                    acc[2] += double(out.re) * double(out.re) + double(out.im) * double(out.im);
+                }
+            }
+            Xlast = xo;
+        } else if constexpr (EPI == 2) {
+            const int qk = kc - L.p0[2];
+            const bool g = live && gcol && qk >= 0 && qk < L.pn[2];
+            const double v = double(gu.re) * double(out.re) + double(gu.im) * double(out.im);
+            double* dst = g ? &L.gg[gcol_off + long(qk) * gplane] : reinterpret_cast<double*>(&L.sink[sink_slot]);
+            *dst = gv + L.gscale * v;
+        } else {
+            cpx<R> r = L.alpha * out;
+            if constexpr (EPI == 1) r = rfma(r, L.beta, bb);
+            cpx<R>* dst = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
+            *dst = r;
+            if (live) acc[0] += double(r.re) * double(r.re) + double(r.im) * double(r.im);
+        }
+    };
+
+    // Steps for planes k0-1 .. k1 (rounded up to a multiple of PF; a trailing
+    // padding step computes but emits nothing).  No memory operation in the
+    // loop body is conditional on the step, so the compiler can keep PF sets
+    // of loads in flight across the back-edge.
+    Set A{}, B{};
+    fetch(k0 - 1, A);
+    if constexpr (PF == 2) fetch(k0, B);
+    const int ntrips = (k1 - k0 + 2 + PF - 1) / PF;
+    for (int tr = 0; tr < ntrips; ++tr) {
+        const int q = k0 - 1 + tr * PF;
+        step(q, tr * PF, A);
+        if constexpr (PF == 2) step(q + 1, tr * PF + 1, B);
+    }
+
+    // optional per-block partials (deterministic two-stage reduction):
+    // part[q*G + lin] for q < NV
+    if (L.norm_part) {
+        __shared__ double red[NV][NT / 64];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double v = acc[q];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if ((tid & 63) == 0) red[q][tid >> 6] = v;
+        }
+        __syncthreads();
+        if (tid < NV) {
+            double s = 0;
+            for (int w = 0; w < NT / 64; ++w) s += red[tid][w];
+            L.norm_part[long(tid) * G + lin] = s;
+        }
+    }
+}
+
+
+}  // namespace
+}  // namespace helm3d

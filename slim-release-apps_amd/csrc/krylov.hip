@@ -17,59 +17,14 @@
 #include <type_traits>
 
 #include "helm27.hpp"
+#include "kbody.hpp"
 #include "krylov.hpp"
 
 namespace helm3d {
 
 namespace {
 
-constexpr int KB = 256;  // threads per block
-
-__device__ inline double wave_sum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Block-wide sums of NV values per thread; results valid in thread 0.
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double* red /* [KB/64][NV] */)
-{
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
-    if (lane == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) red[wid * NV + q] = v[q];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) {
-            double s = red[q];
-            for (int w = 1; w < KB / 64; ++w) s += red[w * NV + q];
-            v[q] = s;
-        }
-    }
-}
-
-// Contiguous range [lo, hi) of this block (even length, for paired points).
-__device__ inline void block_range(long n, long& lo, long& hi)
-{
-    const long per = ((n + gridDim.x - 1) / gridDim.x + 1) & ~1L;
-    lo = long(blockIdx.x) * per;
-    hi = min(n, lo + per);
-}
-
-template <typename R>
-__device__ inline void acc_dot(double& re, double& im, cpx<R> v, cpx<R> w)
-{
-    // += conj(v) * w
-    re = fma(double(v.re), double(w.re), re);
-    re = fma(double(v.im), double(w.im), re);
-    im = fma(double(v.re), double(w.im), im);
-    im = fma(-double(v.im), double(w.re), im);
-}
+using kb::KB;
 
 // d_c = sum conj(v_c) w, c < K.
 template <typename R, int K>
@@ -77,37 +32,7 @@ __global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, 
                                                   const cpx<R>* __restrict__ w, long n,
                                                   double* __restrict__ part)
 {
-    __shared__ double red[(KB / 64) * 2 * K];
-    double acc[2 * K];
-#pragma unroll
-    for (int q = 0; q < 2 * K; ++q) acc[q] = 0.0;
-    long lo, hi;
-    block_range(n, lo, hi);
-    long t = lo + 2 * threadIdx.x;
-    for (; t + 1 < hi; t += 2 * KB) {
-        const cpx<R> w0 = w[t], w1 = w[t + 1];
-        cpx<R> v0[K], v1[K];
-#pragma unroll
-        for (int c = 0; c < K; ++c) {
-            v0[c] = V[c * ldv + t];
-            v1[c] = V[c * ldv + t + 1];
-        }
-#pragma unroll
-        for (int c = 0; c < K; ++c) {
-            acc_dot(acc[2 * c], acc[2 * c + 1], v0[c], w0);
-            acc_dot(acc[2 * c], acc[2 * c + 1], v1[c], w1);
-        }
-    }
-    if (t < hi) {
-        const cpx<R> w0 = w[t];
-#pragma unroll
-        for (int c = 0; c < K; ++c) acc_dot(acc[2 * c], acc[2 * c + 1], V[c * ldv + t], w0);
-    }
-    block_sum<2 * K>(acc, red);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < 2 * K; ++q) part[long(q) * gridDim.x + blockIdx.x] = acc[q];
-    }
+    kb::dots_body<R, K>(V, ldv, w, n, part, blockIdx.x, gridDim.x);
 }
 
 // w <- w - sum_{c<K} a_c v_c with a_c = scl_c * d_c (d: reduced dots in device
@@ -117,49 +42,13 @@ __global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V
                                                     const double* __restrict__ d, CoefPack scl,
                                                     cpx<R>* __restrict__ w, long n, double* __restrict__ part)
 {
-    __shared__ double red[KB / 64];
-    double s2[1] = {0.0};
     double ar[K], ai[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
         ar[c] = scl.v[c] * d[2 * c];
         ai[c] = scl.v[c] * d[2 * c + 1];
     }
-    auto upd = [&](cpx<R> wt, const cpx<R>(&v)[K]) {
-        double re = wt.re, im = wt.im;
-#pragma unroll
-        for (int c = 0; c < K; ++c) {
-            re = fma(-ar[c], double(v[c].re), re);
-            re = fma(ai[c], double(v[c].im), re);
-            im = fma(-ar[c], double(v[c].im), im);
-            im = fma(-ai[c], double(v[c].re), im);
-        }
-        const cpx<R> o{R(re), R(im)};
-        s2[0] += double(o.re) * double(o.re) + double(o.im) * double(o.im);
-        return o;
-    };
-    long lo, hi;
-    block_range(n, lo, hi);
-    long t = lo + 2 * threadIdx.x;
-    for (; t + 1 < hi; t += 2 * KB) {
-        const cpx<R> w0 = w[t], w1 = w[t + 1];
-        cpx<R> v0[K], v1[K];
-#pragma unroll
-        for (int c = 0; c < K; ++c) {
-            v0[c] = V[c * ldv + t];
-            v1[c] = V[c * ldv + t + 1];
-        }
-        w[t] = upd(w0, v0);
-        w[t + 1] = upd(w1, v1);
-    }
-    if (t < hi) {
-        cpx<R> v0[K];
-#pragma unroll
-        for (int c = 0; c < K; ++c) v0[c] = V[c * ldv + t];
-        w[t] = upd(w[t], v0);
-    }
-    block_sum<1>(s2, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s2[0];
+    kb::update_body<R, K>(V, ldv, ar, ai, w, n, part, blockIdx.x, gridDim.x);
 }
 
 // x <- x + sum_{c<K} a_c v_c (the solution update of FGMRES.m:115-119).
@@ -167,41 +56,21 @@ template <typename R, int K>
 __global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ V, long ldv, CoefPack a,
                                                      cpx<R>* __restrict__ x, long n)
 {
-    long lo, hi;
-    block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
-        cpx<R> v[K];
+    double pr[K], pi[K];
 #pragma unroll
-        for (int c = 0; c < K; ++c) v[c] = V[c * ldv + t];
-        const cpx<R> xt = x[t];
-        double re = xt.re, im = xt.im;
-#pragma unroll
-        for (int c = 0; c < K; ++c) {
-            const double pr = a.v[2 * c], pi = a.v[2 * c + 1];
-            re = fma(pr, double(v[c].re), re);
-            re = fma(-pi, double(v[c].im), re);
-            im = fma(pr, double(v[c].im), im);
-            im = fma(pi, double(v[c].re), im);
-        }
-        x[t] = cpx<R>{R(re), R(im)};
+    for (int c = 0; c < K; ++c) {
+        pr[c] = a.v[2 * c];
+        pi[c] = a.v[2 * c + 1];
     }
+    kb::combine_body<R, K>(V, ldv, pr, pi, x, n, blockIdx.x, gridDim.x);
 }
 
 // out[q] = sum_b part[q*nblk + b] in block order; one block per column q.
 __global__ __launch_bounds__(KB) void reduce_kernel(const double* __restrict__ part, int nblk,
                                                     double* __restrict__ out)
 {
-    __shared__ double red[KB];
-    const int q = blockIdx.x;
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += KB) s += part[long(q) * nblk + b];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = KB / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[q] = red[0];
+    const double s = kb::reduce_body(part, nblk, blockIdx.x);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // y = alpha x + beta y (complex alpha, beta); use_y == 0 ignores y's contents.
@@ -228,16 +97,7 @@ template <typename R>
 __global__ __launch_bounds__(KB) void norm2_kernel(const cpx<R>* __restrict__ x, long n,
                                                    double* __restrict__ part)
 {
-    __shared__ double red[KB / 64];
-    double s2[1] = {0.0};
-    long lo, hi;
-    block_range(n, lo, hi);
-    for (long t = lo + threadIdx.x; t < hi; t += KB) {
-        const cpx<R> v = x[t];
-        s2[0] += double(v.re) * double(v.re) + double(v.im) * double(v.im);
-    }
-    block_sum<1>(s2, red);
-    if (threadIdx.x == 0) part[blockIdx.x] = s2[0];
+    kb::norm2_body<R>(x, n, part, blockIdx.x, gridDim.x);
 }
 
 // Split real/imaginary arrays (the MEX boundary layout) <-> interleaved complex.

@@ -35,6 +35,9 @@ HELM3D_C64 = 1
 
 HELM3D_OPT_GRAPHS = 1
 
+HELM3D_BATCH_PLAIN = 0
+HELM3D_BATCH_MIXED = 1
+
 HELM3D_GRID_PHYS = 0
 HELM3D_GRID_PML = 1
 
@@ -102,6 +105,12 @@ _SIGS = [
     ("helm3d_extension_adjoint", ctypes.c_int, [_vp, _ip, _ip, _vp, _vp, ctypes.c_int]),
     ("helm3d_csr_apply", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp,
                                         ctypes.c_int]),
+    ("helm3d_batch_create", ctypes.c_int, [ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    ("helm3d_batch_destroy", ctypes.c_int, [_vp]),
+    ("helm3d_batch_set_stream", ctypes.c_int, [_vp, _vp]),
+    ("helm3d_batch_solve", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int,
+                                          ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
+                                          ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
     ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),

@@ -1,0 +1,69 @@
+"""Batched solves over the C ABI: many right-hand sides on one or several
+operators (grids, frequencies, models) advanced together, one launch per
+kernel type and Krylov step for the whole batch (``helm3d_batch_solve``).
+
+``method="plain"`` is the multi-right-hand-side GMRES of
+tools/solvers/Krylov/FGMRES_MT.m:40-135 in complex128 (lockstep cycles,
+max-over-items stopping); ``method="mixed"`` is FP64 flexible GMRES with a
+fixed number of complex64 BiCGStab iterations as preconditioner, items
+stopping individually.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .core import Helm3d, SolveInfo, _stream_ptr
+
+
+class BatchSolver:
+    """Owns the batch workspace (Krylov bases and inner vectors of every item)
+    on one device; reusable across calls."""
+
+    def __init__(self, device=None):
+        device = torch.device(device if device is not None else ("cuda", torch.cuda.current_device()))
+        self.device = device
+        self._b = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            check(lib().helm3d_batch_create(device.index if device.index is not None else -1,
+                                            ctypes.c_void_p(_stream_ptr()), ctypes.byref(self._b)))
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            lib().helm3d_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, ops: Sequence[Helm3d], bs: Sequence[torch.Tensor], mode: int = 1, method: str = "mixed",
+              tol: float = 1e-6, maxinnerit: int = 20, maxit: int = 10000, inner_maxit: int = 25,
+              raise_on_notconv: bool = False) -> Tuple[List[torch.Tensor], List[SolveInfo]]:
+        """x_i = A_i^-1 b_i for every item (A_i = ops[i] or its adjoint when
+        mode != 1).  ops may repeat the same operator for several
+        right-hand sides."""
+        if len(ops) != len(bs) or not ops:
+            raise ValueError("need one operator per right-hand side")
+        meth = {"plain": _lib.HELM3D_BATCH_PLAIN, "mixed": _lib.HELM3D_BATCH_MIXED}[method]
+        bv = [H._vec(b, torch.complex128) for H, b in zip(ops, bs)]
+        xs = [torch.empty_like(b) for b in bv]
+        n = len(ops)
+        ctxs = (ctypes.c_void_p * n)(*[H._ctx.value for H in ops])
+        bp = (ctypes.c_void_p * n)(*[b.data_ptr() for b in bv])
+        xp = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
+        o = Helm3d._opts(tol, maxinnerit, maxit, False, inner_tol=0.0, inner_maxit=inner_maxit)
+        infos = (_lib.SolveInfo * n)()
+        with torch.cuda.device(self.device):
+            check(lib().helm3d_batch_set_stream(self._b, ctypes.c_void_p(_stream_ptr())))
+            code = lib().helm3d_batch_solve(self._b, n, ctxs, int(mode), bp, xp, meth, ctypes.byref(o), infos)
+        check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
+        out = [SolveInfo(int(i.iters), int(i.cycles), float(i.relres), float(i.quasi_relres), bool(i.converged),
+                         inner_iters=int(i.inner_iters)) for i in infos]
+        return xs, out

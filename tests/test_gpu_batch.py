@@ -1,0 +1,128 @@
+"""GPU batched solves (helm3d_batch_solve) against the CPU oracle.
+
+Bars: every item reaches ||b - A x|| / ||b|| < tol measured with the oracle
+operator in FP64; the plain method (FGMRES_MT.m semantics) keeps all items in
+lockstep (equal iteration counts) and, for one item, takes the iteration count
+of the oracle's FGMRES.m restatement (within 2% + 2); mixed-precision items
+stay within 1e-3 of the complex128 solution; a zero right-hand side returns
+zeros; repeated batch solves are bit-identical.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _op(n, npml, freq, seed):
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small, extend, wavenumber, vec
+    cfg = config_small(n=n, npml=npml, freq=freq, seed=seed)
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to("cuda:0"))
+    return cfg, wn, H
+
+
+def _rhs(cfg, src):
+    from helm3d.models import point_source
+    return point_source(cfg, src)
+
+
+def _relres(cfg, wn, b, x, mode):
+    v = "forw" if mode == 1 else "adj"
+    r = b - O.hmvp(v, wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def test_plain_single_item_matches_fgmres():
+    import torch
+    from helm3d import BatchSolver
+    cfg, wn, H = _op((32, 32, 24), 8, 5.0, 11)
+    b = _rhs(cfg, cfg.sources[0])
+    print("plain single", cfg.nt, flush=True)
+    _, info_o = O.fgmres("forw", wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), b, tol=1e-6, maxinnerit=20)
+    S = BatchSolver("cuda:0")
+    xs, infos = S.solve([H], [torch.from_numpy(b).cuda()], method="plain", tol=1e-6)
+    x = xs[0].cpu().numpy()
+    rel = _relres(cfg, wn, b, x, 1)
+    print("batch iters", infos[0].iters, "oracle", info_o["iters"], "relres", rel, infos[0].relres)
+    assert infos[0].converged and rel < 1e-6
+    assert abs(rel - infos[0].relres) <= 1e-9
+    assert abs(infos[0].iters - info_o["iters"]) <= 0.02 * info_o["iters"] + 2
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_plain_lockstep_uneven_grids(mode):
+    import torch
+    from helm3d import BatchSolver
+    c1, w1, H1 = _op((32, 32, 24), 8, 5.0, 12)
+    c2, w2, H2 = _op((40, 28, 30), 6, 6.0, 13)
+    items = [(c1, w1, H1, (16, 16, 3)), (c1, w1, H1, (5, 20, 10)), (c2, w2, H2, (20, 14, 3))]
+    bs = [_rhs(c, s) for c, _, _, s in items]
+    print("plain lockstep mode", mode, flush=True)
+    S = BatchSolver("cuda:0")
+    xs, infos = S.solve([it[2] for it in items], [torch.from_numpy(b).cuda() for b in bs], mode=mode,
+                        method="plain", tol=1e-6)
+    for (c, w, _, _), b, x, inf in zip(items, bs, xs, infos):
+        rel = _relres(c, w, b, x.cpu().numpy(), mode)
+        print("item", inf.iters, inf.cycles, inf.relres, rel)
+        assert inf.converged and rel < 1e-6
+    assert len({inf.iters for inf in infos}) == 1  # lockstep (FGMRES_MT.m)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mixed_batch_uneven_grids(mode):
+    import torch
+    from helm3d import BatchSolver
+    c1, w1, H1 = _op((32, 32, 24), 8, 5.0, 14)
+    c2, w2, H2 = _op((48, 36, 28), 8, 7.0, 15)
+    items = [(c1, w1, H1, (16, 16, 3)), (c2, w2, H2, (24, 18, 3)), (c2, w2, H2, (8, 30, 20)),
+             (c1, w1, H1, (3, 3, 3))]
+    bs = [_rhs(c, s) for c, _, _, s in items]
+    print("mixed batch mode", mode, flush=True)
+    S = BatchSolver("cuda:0")
+    xs, infos = S.solve([it[2] for it in items], [torch.from_numpy(b).cuda() for b in bs], mode=mode,
+                        method="mixed", tol=1e-6, inner_maxit=10)
+    for (c, w, H, _), b, x, inf in zip(items, bs, xs, infos):
+        xn = x.cpu().numpy()
+        rel = _relres(c, w, b, xn, mode)
+        xr, _ = H.solve(torch.from_numpy(b).cuda(), mode=mode, tol=1e-10)
+        err = np.linalg.norm(xn - xr.cpu().numpy()) / np.linalg.norm(xr.cpu().numpy())
+        print("item iters", inf.iters, "inner", inf.inner_iters, "relres", inf.relres, rel, "err", err)
+        assert inf.converged and rel < 1e-6
+        assert err < 1e-3
+
+
+def test_mixed_single_matches_unbatched_mixed():
+    """A one-item batch (fused complex64 BiCGStab) and the single-vector mixed
+    solve (unfused kernels) converge to the same solution."""
+    import torch
+    from helm3d import BatchSolver
+    cfg, wn, H = _op((32, 32, 24), 8, 5.0, 16)
+    b = torch.from_numpy(_rhs(cfg, cfg.sources[0])).cuda()
+    S = BatchSolver("cuda:0")
+    xs, infos = S.solve([H], [b], method="mixed", tol=1e-8, inner_maxit=12)
+    x1, i1 = H.solve_mixed(b, tol=1e-8, inner_tol=0.0, inner_maxit=12)
+    print("batched", infos[0].iters, "single", i1.iters, flush=True)
+    d = (xs[0] - x1).norm() / x1.norm()
+    assert infos[0].converged and i1.converged
+    assert float(d) < 1e-6
+    # the fused inner solve is the same iteration: same outer step count
+    assert abs(infos[0].iters - i1.iters) <= 1
+
+
+def test_zero_rhs_and_determinism():
+    import torch
+    from helm3d import BatchSolver
+    cfg, wn, H = _op((32, 32, 24), 8, 5.0, 17)
+    b = torch.from_numpy(_rhs(cfg, cfg.sources[0])).cuda()
+    z = torch.zeros_like(b)
+    S = BatchSolver("cuda:0")
+    for method in ("plain", "mixed"):
+        xa, ia = S.solve([H, H], [b, z], method=method, tol=1e-6, inner_maxit=8)
+        xb, ib = S.solve([H, H], [b, z], method=method, tol=1e-6, inner_maxit=8)
+        assert ia[1].converged and ia[1].relres == 0.0 and not xa[1].any()
+        assert torch.equal(xa[0], xb[0])
+        assert ia[0].iters == ib[0].iters
