@@ -10,10 +10,24 @@ h = 10 m, 6 Hz, velocity 1500->4000 m/s linear in depth with a seeded smooth
 resident in HBM before the timed region.
 
 value = solved RHS/s over the timed steps.  The JSON line also carries
-  roofline      the stencil (matvec) kernel: algorithmic bytes (40 B/pt for
-                the FP64 matvec, 56 B/pt for the fused residual) over its
-                average launch duration, timed with HIP events on the
-                library's stream inside the timed solves;
+  roofline      the kernel class that takes most of the solve's time (the
+                classical Gram-Schmidt update w -= V h with ||w||^2,
+                16 (K+2) B/pt for K basis vectors): algorithmic bytes over
+                its launch durations, timed with HIP events on the library's
+                stream inside the timed solves; traffic from the rocprofv3
+                FETCH_SIZE / WRITE_SIZE summary in profiles/;
+  stencil_roofline  the same for the stencil (40 B/pt matvec, 56 B/pt
+                fused residual);
+  mixed_batched the batched mixed-precision solve of the same model with 8
+                sources on a surface line (FP64 flexible GMRES(20), 20
+                complex64 BiCGStab iterations per outer step, all 8 right-hand
+                sides in one launch per kernel type): RHS/s, ratio to the
+                plain path, and each solution's error against the complex128
+                solution of the same source;
+  fwi_grads_per_min  least-squares FWI gradients w.r.t. squared slowness on
+                256x256x128 + 20-pt PML at 12.5 m (16 sources, 64x64
+                receivers, 4 and 6 Hz; batched mixed solves, tol 1e-6): one
+                full gradient (32 forward + 32 adjoint solves) timed;
   cpu_baseline  the reference's own C stencil (do_Hmvp compiled from the
                 reference header in place, OpenMP) driving a C restatement of
                 FGMRES.m on the host cores, timed on one restart cycle of the
@@ -58,11 +72,11 @@ def _alg_bytes(kind, iters, cycles, m, N, stencil_launches):
     return 0.0
 
 
-def _traffic_from_profile():
-    """HBM bytes per stencil launch measured with rocprofv3 PMC counters
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), as
-    recorded in profiles/stencil_fwd_c2_pmc.txt; None when absent."""
-    path = os.path.join(REPO, "profiles", "stencil_fwd_c2_pmc.txt")
+def _traffic_from_profile(name):
+    """HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), as recorded in
+    profiles/<name>; None when absent."""
+    path = os.path.join(REPO, "profiles", name)
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read(), re.M)
@@ -105,6 +119,61 @@ def cpu_baseline(cfg, wn, b, gpu_iters, sample_iters):
         "matvec_ms": mv * 1e3,
         "matvec_gpts": wn.size / mv / 1e9,
     }
+
+
+def mixed_leg(plain_rhs_per_s):
+    """8 sources on the benchmark model, batched mixed precision, versus the
+    complex128 solutions of the same sources."""
+    import torch
+    from helm3d import BatchSolver, Helm3d
+    from helm3d.models import config_c3, extend, wavenumber, vec, point_source
+    cfg = config_c3()
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    dev = torch.device("cuda:0")
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    bs = [torch.from_numpy(point_source(cfg, s)).to(dev) for s in cfg.sources]
+    S = BatchSolver(dev)
+    kw = dict(method="mixed", tol=1e-6, maxinnerit=20, inner_maxit=20)
+    S.solve([H] * len(bs), bs, **kw)  # warm-up (allocation, graph capture)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs, infos = S.solve([H] * len(bs), bs, **kw)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    errs = []
+    for b, x in zip(bs, xs):
+        xr, ir = H.solve(b, tol=1e-6)
+        errs.append(float((x - xr).norm() / xr.norm()))
+    rate = len(bs) / dt
+    return {"workload": "8 point sources (z=5, y=100, x=round(linspace(20,179,8))) on the 240x240x140 benchmark "
+                        "model, one batched mixed-precision solve to 1e-6 FP64 residual",
+            "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s,
+            "outer_iters": [i.iters for i in infos], "inner_iters_per_outer": 20,
+            "final_relres": [i.relres for i in infos], "err_vs_complex128": errs,
+            "dtype": "c64 inner / f64 outer"}
+
+
+def fwi_leg():
+    """One least-squares FWI gradient on the 256x256x128 workload."""
+    import torch
+    from helm3d.fwi import FWIOpts, FWIProblem
+    from helm3d.models import fwi_c5
+    model, m_true, m0 = fwi_c5()
+    dev = torch.device("cuda:0")
+    P = FWIProblem(model, FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20),
+                   device=dev)
+    D = P.forw_model(m_true)  # observed data (not timed)
+    P.opts.stats.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f, g = P.objective(m0, D)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    its = [i.iters for i in P.opts.stats]
+    return 60.0 / dt, {"workload": "LS-FWI gradient w.r.t. s2/m2, 256x256x128 + 20-pt PML at 12.5 m, 16 sources, "
+                                   "64x64 receivers, 4 and 6 Hz, batched mixed solves to 1e-6",
+                       "seconds_per_gradient": dt, "solves": len(its), "outer_iters_mean": sum(its) / len(its),
+                       "misfit": f, "grad_norm": float(g.norm())}
 
 
 def copy_mode(args):
@@ -159,6 +228,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-precision leg")
+    ap.add_argument("--no-fwi", action="store_true", help="skip the FWI gradient leg")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
     ap.add_argument("--mode", choices=("solve", "matvec", "copy"), default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
@@ -213,14 +284,33 @@ def main():
     cycles = infos[-1].cycles
     st_ms, st_n = prof["stencil"]
     per_solve_launches = st_n / args.steps
-    st_bytes = _alg_bytes("stencil", iters, cycles, m, N, per_solve_launches) * args.steps
-    achieved = st_bytes / (st_ms * 1e-3) / 1e9
     breakdown = {}
     for kind, (ms, n) in prof.items():
         byt = _alg_bytes(kind, iters, cycles, m, N, per_solve_launches) * args.steps
         breakdown[kind] = {"ms_per_solve": ms / args.steps, "launches_per_solve": n / args.steps,
+                           "alg_bytes": byt, "total_ms": ms, "launches": n,
                            "GB/s": (byt / (ms * 1e-3) / 1e9) if ms > 0 and byt > 0 else None}
-    traffic = _traffic_from_profile()
+
+    def roof(kind, kernel, profile):
+        b = breakdown[kind]
+        ach = b["alg_bytes"] / (b["total_ms"] * 1e-3) / 1e9
+        nl = max(1, b["launches"])
+        tr = _traffic_from_profile(profile)
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": tr, "kernel": kernel,
+                "alg_bytes_per_launch": b["alg_bytes"] / nl, "avg_launch_ms": b["total_ms"] / nl,
+                "share_of_solve_kernel_time": b["total_ms"] / sum(v["total_ms"] for v in breakdown.values())}
+
+    dominant = max(("update", "dots", "stencil"), key=lambda k: breakdown[k]["total_ms"])
+    kernels = {"update": "update_kernel<double,K>: CGS update w -= V h and ||w||^2, 16 (K+2) B/pt",
+               "dots": "dots_kernel<double,K>: CGS dots V^H w, 16 (K+1) B/pt",
+               "stencil": "helm27 stencil, FP64 forward matvec (40 B/pt) + fused residual (56 B/pt)"}
+    profiles = {"update": "gs_update_c2_pmc.txt", "dots": "gs_dots_c2_pmc.txt", "stencil": "stencil_fwd_c2_pmc.txt"}
+    roofline = roof(dominant, kernels[dominant], profiles[dominant])
+    stencil_roofline = roof("stencil", kernels["stencil"], "stencil_fwd_c2_pmc.txt")
+    for v in breakdown.values():
+        for k in ("alg_bytes", "total_ms", "launches"):
+            v.pop(k)
 
     out = {
         "metric": METRIC,
@@ -240,19 +330,23 @@ def main():
                         "200x200x100 + 20-pt PML (240x240x140), h=10 m, 6 Hz, 1 point source",
             "grid_points": N, "restart": m, "tol": 1e-6, "precision": "complex128",
         },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": "helm27 stencil, FP64 forward matvec (40 B/pt) + fused residual (56 B/pt)",
-            "avg_launch_ms": st_ms / max(1, st_n),
-        },
-        "matvec_pct_hbm_peak": 100.0 * achieved / HBM_PEAK_GBS,
+        "roofline": roofline,
+        "stencil_roofline": stencil_roofline,
+        "matvec_pct_hbm_peak": 100.0 * stencil_roofline["frac"],
         "iters_per_solve": iters,
         "cycles_per_solve": cycles,
         "final_relres": infos[-1].relres,
         "kernel_breakdown": breakdown,
         "fwi_grads_per_min": None,
     }
+    del H
+    torch.cuda.empty_cache()
+    if not args.no_mixed:
+        out["mixed_batched"] = mixed_leg(args.steps / elapsed)
+        print(f"[bench] mixed leg done: {out['mixed_batched']['rhs_per_s']:.3f} RHS/s", file=sys.stderr, flush=True)
+    if not args.no_fwi:
+        out["fwi_grads_per_min"], out["fwi"] = fwi_leg()
+        print(f"[bench] fwi leg done: {out['fwi_grads_per_min']:.3f} grads/min", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, wn, b, iters, args.cpu_sample_iters)
     print(json.dumps(out))

@@ -80,7 +80,9 @@ class FWIOpts:
 
     solver: "fgmres" (FGMRES.m in complex128) or "mixed" (FP64 flexible GMRES
     with a complex64 BiCGStab preconditioner); tol is the relative residual of
-    every solve."""
+    every solve.  batch > 1 solves that many sources of a frequency together
+    (helm3d_batch_solve: FGMRES_MT.m semantics for "fgmres", a fixed
+    inner_iters complex64 BiCGStab iterations per outer step for "mixed")."""
     pml: Optional[object] = 10
     pml_max: int = 10
     cut_pml: bool = True
@@ -94,6 +96,8 @@ class FWIOpts:
     maxit: int = 10000
     inner_tol: float = 1e-2
     inner_maxit: int = 200
+    batch: int = 1
+    inner_iters: int = 20
     stats: list = field(default_factory=list, repr=False)
 
 
@@ -180,6 +184,7 @@ class FWIProblem:
         if self.Q.shape[0] != model.nsrc:
             raise ValueError("size(Q,1) must match the source grid")
         self.H = Helm3d(device=self.device)
+        self._batch = None
         self._geom = {}   # npml key -> (Ps, Pr, Pr^T)
 
     # -- grids and operators --------------------------------------------------
@@ -247,6 +252,24 @@ class FWIProblem:
             raise ValueError("solver must be 'fgmres' or 'mixed'")
         o.stats.append(info)
         return x
+
+    def _solve_many(self, bs, mode: int):
+        """Solves for a list of right-hand sides, opts.batch at a time."""
+        o = self.opts
+        if o.batch <= 1:
+            return [self._solve(b, mode) for b in bs]
+        from .batch import BatchSolver
+        if self._batch is None:
+            self._batch = BatchSolver(self.device)
+        method = {"fgmres": "plain", "mixed": "mixed"}[o.solver]
+        out = []
+        for i in range(0, len(bs), o.batch):
+            chunk = bs[i:i + o.batch]
+            xs, infos = self._batch.solve([self.H] * len(chunk), chunk, mode=mode, method=method, tol=o.tol,
+                                          maxinnerit=o.maxinnerit, maxit=o.maxit, inner_maxit=o.inner_iters)
+            o.stats.extend(infos)
+            out.extend(xs)
+        return out
 
     def _source(self, S, k: int, isrc: int) -> torch.Tensor:
         """Qk_i = Ps (w(f) Q(:, isrc)) prod(d)/prod(dt) (PDEfunc.m:324-336);
@@ -328,8 +351,8 @@ class FWIProblem:
         Us = []
         for k, freq in enumerate(self.model.freq):
             S = self._setup(v, freq)
-            for j in range(nsrc):
-                U = self._solve(self._source(S, k, j), 1)
+            Uk = self._solve_many([self._source(S, k, j) for j in range(nsrc)], 1)
+            for j, U in enumerate(Uk):
                 pred[:, k * nsrc + j] = self._sample(S, U)
                 Us.append(U if keep else None)
         return pred, Us
@@ -345,15 +368,16 @@ class FWIProblem:
         for k, freq in enumerate(self.model.freq):
             S = self._setup(v, freq)
             gpml = self._grad_buffers(S) if need_grad else None
-            for j in range(nsrc):
-                col = k * nsrc + j
-                U = self._solve(self._source(S, k, j), 1)
-                r = self._sample(S, U) - D[:, col]
+            Us = self._solve_many([self._source(S, k, j) for j in range(nsrc)], 1)
+            rs = []
+            for j, U in enumerate(Us):
+                r = self._sample(S, U) - D[:, k * nsrc + j]
                 f += 0.5 * float(torch.vdot(r, r).real)
-                if need_grad:
-                    V = self._solve(self._inject_rec(S, r, -1.0), 2)
-                    self._grad_term(S, freq, U, V, g, gpml)
+                rs.append(r)
             if need_grad:
+                Vs = self._solve_many([self._inject_rec(S, r, -1.0) for r in rs], 2)
+                for U, V in zip(Us, Vs):
+                    self._grad_term(S, freq, U, V, g, gpml)
                 self._grad_finish(S, g, gpml)
         if need_grad:
             self._zero_boundary(g)
@@ -390,9 +414,9 @@ class FWIProblem:
         for k, freq in enumerate(self.model.freq):
             S = self._setup(v, freq)
             gpml = self._grad_buffers(S)
-            for j in range(nsrc):
-                U = self._solve(self._source(S, k, j), 1)
-                V = self._solve(self._inject_rec(S, dd[:, k * nsrc + j], -1.0), 2)
+            Us = self._solve_many([self._source(S, k, j) for j in range(nsrc)], 1)
+            Vs = self._solve_many([self._inject_rec(S, dd[:, k * nsrc + j], -1.0) for j in range(nsrc)], 2)
+            for U, V in zip(Us, Vs):
                 self._grad_term(S, freq, U, V, g, gpml)
             self._grad_finish(S, g, gpml)
         return g
@@ -464,12 +488,13 @@ class FWIObjective(torch.autograd.Function):
         Us, R = [], []
         for k, freq in enumerate(P.model.freq):
             S = P._setup(v, freq)
-            for j in range(nsrc):
-                U = P._solve(P._source(S, k, j), 1)
+            Uk = P._solve_many([P._source(S, k, j) for j in range(nsrc)], 1)
+            for j, U in enumerate(Uk):
                 r = P._sample(S, U) - D[:, k * nsrc + j]
                 f += 0.5 * float(torch.vdot(r, r).real)
                 Us.append(U if keep else None)
                 R.append(r)
+            del Uk
         ctx.problem = P
         ctx.Us = Us
         ctx.R = R
@@ -488,14 +513,20 @@ class FWIObjective(torch.autograd.Function):
         for k, freq in enumerate(P.model.freq):
             S = P._setup(v, freq)
             gpml = P._grad_buffers(S)
-            for j in range(nsrc):
-                idx = k * nsrc + j
-                U = ctx.Us[idx]
-                if U is None:
-                    U = P._solve(P._source(S, k, j), 1)
-                V = P._solve(P._inject_rec(S, ctx.R[idx], -1.0), 2)
-                P._grad_term(S, freq, U, V, g, gpml)
-                ctx.Us[idx] = None
+            # sources in chunks of the batch size: wavefields that were not
+            # kept are recomputed one chunk at a time
+            step = max(1, P.opts.batch)
+            for j0 in range(0, nsrc, step):
+                idx = [k * nsrc + j for j in range(j0, min(nsrc, j0 + step))]
+                missing = [i for i in idx if ctx.Us[i] is None]
+                if missing:
+                    Ur = P._solve_many([P._source(S, k, i - k * nsrc) for i in missing], 1)
+                    for i, U in zip(missing, Ur):
+                        ctx.Us[i] = U
+                Vs = P._solve_many([P._inject_rec(S, ctx.R[i], -1.0) for i in idx], 2)
+                for i, V in zip(idx, Vs):
+                    P._grad_term(S, freq, ctx.Us[i], V, g, gpml)
+                    ctx.Us[i] = None
             P._grad_finish(S, g, gpml)
         P._zero_boundary(g)
         return (g * grad_out).reshape(ctx.m_shape).to(ctx.m_dtype), None, None, None

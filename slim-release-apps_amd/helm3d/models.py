@@ -91,6 +91,36 @@ def config_small(n=(32, 32, 24), npml=8, h=10.0, freq=5.0, seed: int = SEED_BASE
     return Config("small", tuple(n), npml, h, freq, v, [(n[0] // 2, n[1] // 2, 3)])
 
 
+def fwi_c5(seed: int = SEED_BASE + 5, n=(256, 256, 128), h: float = 12.5, npml: int = 20,
+           freqs=(4.0, 6.0), nblobs: int = 12):
+    """FWI workload: 256x256x128 at 12.5 m + 20-point PML; true model = smooth
+    1500-4500 m/s background (sigma 20 points) + 12 seeded Gaussian blobs of
+    +-300 m/s with radii 10-25 points; start model = true model smoothed with
+    sigma 20; 16 sources on a 4x4 surface grid and 64x64 receivers at depth
+    index 2; frequencies 4 and 6 Hz; Ricker spectrum with f0 = 10 Hz, t0 = 0.
+    Returns (FWIModel in s2/m2, m_true, m_start) with m on the physical grid
+    (x fastest)."""
+    from .fwi import FWIModel
+    rng = np.random.default_rng(seed)
+    v = _scaled_smooth(n, 20, rng, 1500.0, 4500.0)
+    ax = [np.arange(a, dtype=np.float64) for a in n]
+    for _ in range(nblobs):
+        c = rng.uniform(0, 1, 3) * np.array(n)
+        r = rng.uniform(10.0, 25.0)
+        amp = 300.0 * rng.choice([-1.0, 1.0])
+        gx = np.exp(-((ax[0] - c[0]) / r) ** 2)
+        gy = np.exp(-((ax[1] - c[1]) / r) ** 2)
+        gz = np.exp(-((ax[2] - c[2]) / r) ** 2)
+        v += amp * gx[:, None, None] * gy[None, :, None] * gz[None, None, :]
+    v0 = gaussian_filter(v, sigma=20, mode="reflect")
+    src = np.round(np.linspace(n[0] / 8, n[0] * 7 / 8, 4)) * h
+    rx = (2 + 4 * np.arange(64)) * h
+    model = FWIModel(o=(0.0, 0.0, 0.0), d=(h, h, h), n=n, freq=list(freqs),
+                     xsrc=src, ysrc=src, zsrc=np.array([2 * h]),
+                     xrec=rx, yrec=rx, zrec=np.array([2 * h]), f0=10.0, t0=0.0, unit="s2/m2")
+    return model, vec(1.0 / v ** 2), vec(1.0 / v0 ** 2)
+
+
 def extend(v: np.ndarray, npml: int) -> np.ndarray:
     """Edge-replicating extension into the PML (opExtension mode 1)."""
     return np.pad(v, npml, mode="edge")

@@ -220,3 +220,26 @@ def test_mixed_precision_gradient():
     fo, go = ora.objective(m0, Dobs)
     assert abs(f - fo) <= 1e-8 * abs(fo)
     assert relerr(g.cpu().numpy(), go) <= 1e-6
+
+
+@pytest.mark.parametrize("solver", ["fgmres", "mixed"])
+def test_batched_objective_matches_unbatched(solver):
+    """Sources of a frequency solved together (helm3d_batch_solve) give the
+    objective and gradient of one-at-a-time solves."""
+    import torch
+    from helm3d.fwi import FWIModel, FWIOpts, FWIProblem, fwi_misfit
+    model = FWIModel(o=(0.0, 0.0, 0.0), d=D, n=N_PHYS, freq=[5.0, 6.0], xsrc=np.array([100.0, 200.0, 300.0]),
+                     ysrc=SRC[1], zsrc=SRC[2], xrec=REC[0], yrec=REC[1], zrec=REC[2], f0=10.0, unit="s2/m2")
+    m_true, m0 = _models("s2/m2")
+    P1 = FWIProblem(model, FWIOpts(pml=3, tol=1e-10, maxit=100000, solver=solver), device=torch.device("cuda:0"))
+    Pb = FWIProblem(model, FWIOpts(pml=3, tol=1e-10, maxit=100000, solver=solver, batch=2, inner_iters=10),
+                    device=torch.device("cuda:0"))
+    Dobs = P1.forw_model(m_true)
+    f1, g1 = P1.objective(m0, Dobs)
+    fb, gb = Pb.objective(m0, Dobs)
+    print(f"batched {solver}: f {f1} {fb}", flush=True)
+    assert abs(f1 - fb) <= 1e-8 * abs(f1)
+    assert relerr(gb.cpu().numpy(), g1.cpu().numpy()) <= 1e-7
+    m = torch.from_numpy(m0.copy()).cuda().requires_grad_(True)
+    fwi_misfit(m, Pb, Dobs, wavefields="recompute").backward()
+    assert relerr(m.grad.cpu().numpy(), g1.cpu().numpy()) <= 1e-7
