@@ -133,6 +133,7 @@ def mixed_leg(plain_rhs_per_s):
     H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
     bs = [torch.from_numpy(point_source(cfg, s)).to(dev) for s in cfg.sources]
     S = BatchSolver(dev)
+    tuned = S.autotune([H] * len(bs), bs, inner_maxit=20)  # launch shape of the batched kernels
     kw = dict(method="mixed", tol=1e-6, maxinnerit=20, inner_maxit=20)
     S.solve([H] * len(bs), bs, **kw)  # warm-up (allocation, graph capture)
     torch.cuda.synchronize()
@@ -149,7 +150,7 @@ def mixed_leg(plain_rhs_per_s):
                         "model, one batched mixed-precision solve to 1e-6 FP64 residual",
             "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s,
             "outer_iters": [i.iters for i in infos], "inner_iters_per_outer": 20,
-            "final_relres": [i.relres for i in infos], "err_vs_complex128": errs,
+            "final_relres": [i.relres for i in infos], "err_vs_complex128": errs, "launch": tuned,
             "dtype": "c64 inner / f64 outer"}
 
 
@@ -202,6 +203,8 @@ def matvec_mode(args, cfg, wn):
     from helm3d import Helm3d
     dev = torch.device("cuda:0")
     H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    if not args.no_tune:
+        H.autotune(mode=1, precision="c128")
     x = torch.from_numpy(np.random.default_rng(0).standard_normal(H.N) + 0j).to(dev)
     y = torch.empty_like(x)
     for _ in range(args.warmup):
@@ -230,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-precision leg")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI gradient leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
     ap.add_argument("--mode", choices=("solve", "matvec", "copy"), default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
@@ -260,6 +264,7 @@ def main():
     bt = torch.from_numpy(b).to(dev)
     N = H.N
     m = 20
+    tuned = H.autotune(mode=1, precision="c128") if not args.no_tune else H.get_launch()
 
     for _ in range(args.warmup):
         x, info = H.solve(bt, tol=1e-6, maxinnerit=m)
@@ -328,7 +333,7 @@ def main():
         "config": {
             "workload": "plain complex128 FGMRES(20) solve to 1e-6 of the 27-pt Helmholtz stencil with PML, "
                         "200x200x100 + 20-pt PML (240x240x140), h=10 m, 6 Hz, 1 point source",
-            "grid_points": N, "restart": m, "tol": 1e-6, "precision": "complex128",
+            "grid_points": N, "restart": m, "tol": 1e-6, "precision": "complex128", "stencil_launch": tuned,
         },
         "roofline": roofline,
         "stencil_roofline": stencil_roofline,

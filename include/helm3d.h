@@ -136,6 +136,15 @@ int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *
  * the autotuner of the Python package (helm3d.tune). */
 int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch);
 int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *blocks);
+/* Autotuner of the stencil launch shape for this context's grid: times every
+ * candidate shape (64 x {4, 8} column tiles, prefetch {1, 2}, z chunks of the
+ * whole column down to 8 planes) on the operator of `mode` (1: H, else H^H)
+ * in `precision`, `reps` launches each with HIP events on the context's
+ * stream, and keeps the fastest (results are identical for every shape).  No
+ * counterpart in the reference; the batched solves use each item's tuned z
+ * chunk and the first item's tile shape. */
+int helm3d_autotune(helm3d_ctx *ctx, int mode, int precision, int reps, int *kchunk, int *tile_rows,
+                    int *prefetch, double *best_ms);
 
 /* Context options.  HELM3D_OPT_GRAPHS (default 1): replay fixed kernel
  * sequences (the fixed-length inner solve of helm3d_solve_mixed) from captured

@@ -43,39 +43,6 @@ constexpr int kMaxChunk = 254;  // z planes per block (LDS holds their PML weigh
 template <typename R>
 __device__ inline cpx<R> zero() { return cpx<R>{R(0), R(0)}; }
 
-// Per-column coefficient sets.  For neighbour offset (dx,dy) (a = dx+1,
-// b = dy+1) with side values xs[a], ys[b] (xs[1], ys[1] = full xi):
-//   F (dz = 0):  centre -(c_x xi_x + c_y xi_y); x faces c_x xi_x^s + w_xy xi_y;
-//                y faces c_y xi_y^s + w_xy xi_x; diagonals -w_xy (xi_x^s + xi_y^s)
-//   G (dz=+-1):  centre w_xz xi_x + w_yz xi_y; x faces -w_xz xi_x^s + w3a2 xi_y;
-//                y faces -w_yz xi_y^s + w3a2 xi_x; diagonals -w3a2 (xi_x^s + xi_y^s)
-// i.e. the non-xi_z part of the coefficients of Helm3d_27pt.h:680-714.
-template <typename R>
-__device__ inline void column_coefs(const OpDesc<R>& op, const cpx<R> xs[3], const cpx<R> ys[3],
-                                    cpx<R> F[3][3], cpx<R> G[3][3])
-{
-    const cpx<R> xf = xs[1], yf = ys[1];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            if (a == 1 && b == 1) {
-                F[a][b] = -(op.c[0] * xf + op.c[1] * yf);
-                G[a][b] = op.xz * xf + op.yz * yf;
-            } else if (b == 1) {
-                F[a][b] = op.c[0] * xs[a] + op.xy * yf;
-                G[a][b] = (-op.xz) * xs[a] + op.w3a2 * yf;
-            } else if (a == 1) {
-                F[a][b] = op.c[1] * ys[b] + op.xy * xf;
-                G[a][b] = (-op.yz) * ys[b] + op.w3a2 * xf;
-            } else {
-                F[a][b] = (-op.xy) * (xs[a] + ys[b]);
-                G[a][b] = (-op.w3a2) * (xs[a] + ys[b]);
-            }
-        }
-    }
-}
-
 // FP64 complex scalars of the fused BiCGStab forms (device slots, see
 // bicgstab.hpp); a zero denominator gives zero (the breakdown guard of the
 // BiCGStab kernels).
@@ -120,7 +87,11 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
     const OpDesc<R>& op = L.op;
     __shared__ cpx<R> su[2][HY][HX];            // input vector plane (+halo)
-    __shared__ cpx<R> sm[ADJ ? 1 : 2][HY][HX];  // forward: wn .* x plane (+halo)
+    // forward mass operand: wn .* x (WNC, DERIV) or the real wn itself, the
+    // product being formed from the x plane already read for the stiffness
+    constexpr bool kWnSep = !ADJ && !WNC && !DERIV;
+    __shared__ cpx<R> sm[(ADJ || kWnSep) ? 1 : 2][HY][HX];
+    __shared__ R sw[kWnSep ? 2 : 1][HY][HX];
     __shared__ cpx<R> szl[kMaxChunk + 4], szh[kMaxChunk + 4];  // z PML slots k0 .. k1+3
 
     const int nx = op.nx, ny = op.ny, nz = op.nz;
@@ -150,23 +121,21 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                           jw <= op.int_hi[1];
 
     // ---- per-column coefficients (fixed for the whole march)
-    cpx<R> F[3][3], Gc[3][3];
+    // Every F and G coefficient is a real combination of the six x/y PML
+    // values of the column, xs[0], xs[2], ys[0], ys[2] (neighbour sides) and
+    // xs[1], ys[1] (the column's own xi), so F.u and G.u are applied in
+    // factored form: six complex multipliers times real-weighted sums of the
+    // plane's 3x3 neighbourhood (24 registers instead of 18 complex
+    // coefficients).  Adjoint: side values of the neighbours, conjugated.
+    cpx<R> px0{}, px2{}, py0{}, py2{}, pxf{}, pyf{};
     if constexpr (!DERIV) {
-        cpx<R> xs[3], ys[3];
         if constexpr (!ADJ) {
-            xs[0] = op.xlo[ic + 1]; xs[2] = op.xhi[ic + 1]; xs[1] = xs[0] + xs[2];
-            ys[0] = op.ylo[jc + 1]; ys[2] = op.yhi[jc + 1]; ys[1] = ys[0] + ys[2];
+            px0 = op.xlo[ic + 1]; px2 = op.xhi[ic + 1]; pxf = px0 + px2;
+            py0 = op.ylo[jc + 1]; py2 = op.yhi[jc + 1]; pyf = py0 + py2;
         } else {
             // neighbour i-1 couples through its hi value, i+1 through its lo value
-            xs[0] = op.xhi[ic]; xs[2] = op.xlo[ic + 2]; xs[1] = op.xlo[ic + 1] + op.xhi[ic + 1];
-            ys[0] = op.yhi[jc]; ys[2] = op.ylo[jc + 2]; ys[1] = op.ylo[jc + 1] + op.yhi[jc + 1];
-        }
-        column_coefs(op, xs, ys, F, Gc);
-        if constexpr (ADJ) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) { F[a][b] = conjc(F[a][b]); Gc[a][b] = conjc(Gc[a][b]); }
+            px0 = conjc(op.xhi[ic]); px2 = conjc(op.xlo[ic + 2]); pxf = conjc(op.xlo[ic + 1] + op.xhi[ic + 1]);
+            py0 = conjc(op.yhi[jc]); py2 = conjc(op.ylo[jc + 2]); pyf = conjc(op.ylo[jc + 1] + op.yhi[jc + 1]);
         }
         // z weights of this chunk: slots k0 .. k1+1 (planes k0-1 .. k1) plus
         // two padding entries read by the trailing pipeline step
@@ -314,8 +283,13 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                 if (has1) sm[buf][s1y][s1x] = cmul(S.c1, x1);
             } else {
                 const R w0 = op.wn ? S.w0 : op.wn_const, w1 = op.wn ? S.w1 : op.wn_const;
-                sm[buf][s0y][s0x] = w0 * x0;
-                if (has1) sm[buf][s1y][s1x] = w1 * x1;
+                if constexpr (kWnSep) {
+                    sw[buf][s0y][s0x] = w0;
+                    if (has1) sw[buf][s1y][s1x] = w1;
+                } else {
+                    sm[buf][s0y][s0x] = w0 * x0;
+                    if (has1) sm[buf][s1y][s1x] = w1 * x1;
+                }
             }
         }
         // own-point operands of plane q-1 leave the set before the refill
@@ -326,6 +300,15 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         fetch(q + PF, S);  // unconditional: a fixed number of loads per step
         __syncthreads();
 
+        // ---- the plane's 3x3 neighbourhood (stiffness operand)
+        cpx<R> u[3][3];
+        if constexpr (!DERIV) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) u[a][b] = su[buf][ty + b][tx + a];
+        }
+
         // ---- mass-lumping sums (forward: on wn .* x, adjoint: on x)
         cpx<R> Mc, Mo;
         {
@@ -333,7 +316,11 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
-                for (int b = 0; b < 3; ++b) m[a][b] = ADJ ? su[buf][ty + b][tx + a] : sm[buf][ty + b][tx + a];
+                for (int b = 0; b < 3; ++b) {
+                    if constexpr (kWnSep) m[a][b] = sw[buf][ty + b][tx + a] * u[a][b];
+                    else if constexpr (ADJ) m[a][b] = DERIV ? su[buf][ty + b][tx + a] : u[a][b];
+                    else m[a][b] = sm[buf][ty + b][tx + a];
+                }
             const cpx<R> mf = (m[0][1] + m[2][1]) + (m[1][0] + m[1][2]);
             const cpx<R> md = (m[0][0] + m[2][0]) + (m[0][2] + m[2][2]);
             Mc = op.wn_coef * m[1][1];
@@ -347,11 +334,6 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         // ---- stiffness: Qz (z PML, applied per plane) and F, G (x/y PML)
         cpx<R> Qn = zero<R>(), Fn = zero<R>(), Gn = zero<R>();
         if constexpr (!DERIV) {
-            cpx<R> u[3][3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) u[a][b] = su[buf][ty + b][tx + a];
             // pair sums along y (per column a) and along x (per row b)
             const cpx<R> sy0 = u[0][0] + u[0][2], sy1 = u[1][0] + u[1][2], sy2 = u[2][0] + u[2][2];
             const cpx<R> sx1 = u[0][1] + u[2][1];
@@ -372,13 +354,43 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                 Gn = rfma(Gn, op.gi[2], sy1);
                 Gn = rfma(Gn, op.gi[3], dg);
             } else {
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) {
-                        Fn = cfma(Fn, F[a][b], u[a][b]);
-                        Gn = cfma(Gn, Gc[a][b], u[a][b]);
-                    }
+                // F: centre -(cx xf + cy yf); x faces cx xs + w_xy yf; y faces
+                //    cy ys + w_xy xf; diagonals -w_xy (xs + ys)
+                // G: centre w_xz xf + w_yz yf; x faces -w_xz xs + w3a2 yf;
+                //    y faces -w_yz ys + w3a2 xf; diagonals -w3a2 (xs + ys)
+                // (the non-xi_z part of Helm3d_27pt.h:680-714), grouped by
+                // multiplier; sy0/sy2: column sums, sx0/sx2: row sums of the
+                // diagonal pairs.
+                const cpx<R> sx0 = u[0][0] + u[2][0], sx2 = u[0][2] + u[2][2];
+                const R cx = op.c[0], cy = op.c[1], wxy = op.xy, wxz = op.xz, wyz = op.yz, w3 = op.w3a2;
+                cpx<R> f0 = cx * u[0][1], f2 = cx * u[2][1], f3 = cy * u[1][0], f5 = cy * u[1][2];
+                f0 = rfma(f0, -wxy, sy0);
+                f2 = rfma(f2, -wxy, sy2);
+                f3 = rfma(f3, -wxy, sx0);
+                f5 = rfma(f5, -wxy, sx2);
+                cpx<R> fy = wxy * sx1, fx = wxy * sy1;
+                fy = rfma(fy, -cy, u[1][1]);
+                fx = rfma(fx, -cx, u[1][1]);
+                cpx<R> g0 = (-wxz) * u[0][1], g2 = (-wxz) * u[2][1], g3 = (-wyz) * u[1][0], g5 = (-wyz) * u[1][2];
+                g0 = rfma(g0, -w3, sy0);
+                g2 = rfma(g2, -w3, sy2);
+                g3 = rfma(g3, -w3, sx0);
+                g5 = rfma(g5, -w3, sx2);
+                cpx<R> gy = w3 * sx1, gx = w3 * sy1;
+                gy = rfma(gy, wyz, u[1][1]);
+                gx = rfma(gx, wxz, u[1][1]);
+                Fn = cmul(px0, f0);
+                Fn = cfma(Fn, px2, f2);
+                Fn = cfma(Fn, py0, f3);
+                Fn = cfma(Fn, py2, f5);
+                Fn = cfma(Fn, pyf, fy);
+                Fn = cfma(Fn, pxf, fx);
+                Gn = cmul(px0, g0);
+                Gn = cfma(Gn, px2, g2);
+                Gn = cfma(Gn, py0, g3);
+                Gn = cfma(Gn, py2, g5);
+                Gn = cfma(Gn, pyf, gy);
+                Gn = cfma(Gn, pxf, gx);
             }
         }
 
@@ -462,7 +474,13 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // of loads in flight across the back-edge.
     Set A{}, B{};
     fetch(k0 - 1, A);
-    if constexpr (PF == 2) fetch(k0, B);
+    if constexpr (PF == 2) {
+        // keep set A's loads ahead of set B's: the loop then waits only for
+        // the set it stages (the wait count at the loop header is merged
+        // from this prologue and the back edge)
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(k0, B);
+    }
     const int ntrips = (k1 - k0 + 2 + PF - 1) / PF;
     for (int tr = 0; tr < ntrips; ++tr) {
         const int q = k0 - 1 + tr * PF;

@@ -700,6 +700,76 @@ int helm3d_get_launch(helm3d_ctx* c, int* kchunk, int* tile_rows, int* prefetch,
     return HELM3D_OK;
 }
 
+int helm3d_autotune(helm3d_ctx* c, int mode, int precision, int reps, int* kchunk, int* tile_rows, int* prefetch,
+                    double* best_ms)
+{
+    CHK(check_model(c));
+    if (precision != HELM3D_C128 && precision != HELM3D_C64) return fail(HELM3D_ERR_ARG, "bad precision");
+    reps = std::max(1, std::min(reps, 1000));
+    const int variant = model_variant(c, mode != 1);
+    const long N = c->N;
+    const long ld = slot_ld(N);
+    // scratch: two vectors of the context's staging buffer (complex128
+    // slots hold the complex64 pair as well)
+    CHK(ensure_vecs(c->T64, N, 2));
+    HIPCHK(hipMemsetAsync(c->T64.p, 0, size_t(2 * ld) * sizeof(cpx<double>), c->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const StencilGrid keep = c->grid;
+    StencilGrid best = keep;
+    double best_t = 1e300;
+    int rc = HELM3D_OK;
+    // candidate z chunks: whole column, halves, ... down to 8 planes
+    std::vector<int> chunks;
+    for (int d : {1, 2, 3, 4, 6, 8, 12}) {
+        const int kc = (c->n[2] + d - 1) / d;
+        if (kc >= 8 || d == 1) chunks.push_back(kc);
+    }
+    for (int ty : {4, 8}) {
+        for (int pf : {1, 2}) {
+            for (int kc : chunks) {
+                c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kc, ty, pf);
+                CHK(ensure_work(c));
+                float ms = 0.f;
+                auto run = [&]() -> int {
+                    if (precision == HELM3D_C128) {
+                        const cpx<double>* x = c->T64.p;
+                        cpx<double>* y = c->T64.p + ld;
+                        return stencil<double>(c, variant, c->op64, x, y, 1.0, nullptr, 0.0, nullptr);
+                    }
+                    const cpx<float>* x = reinterpret_cast<const cpx<float>*>(c->T64.p);
+                    cpx<float>* y = reinterpret_cast<cpx<float>*>(c->T64.p + ld);
+                    return stencil<float>(c, variant, c->op32, x, y, 1.0f, nullptr, 0.0f, nullptr);
+                };
+                for (int w = 0; w < 2 && rc == HELM3D_OK; ++w) rc = run();
+                if (rc == HELM3D_OK && hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(HELM3D_ERR_HIP, "event");
+                for (int r = 0; r < reps && rc == HELM3D_OK; ++r) rc = run();
+                if (rc == HELM3D_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(HELM3D_ERR_HIP, "event");
+                if (rc == HELM3D_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(HELM3D_ERR_HIP, "event sync");
+                if (rc == HELM3D_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(HELM3D_ERR_HIP, "event time");
+                if (rc != HELM3D_OK) break;
+                const double t = double(ms) / reps;
+                if (t < best_t) {
+                    best_t = t;
+                    best = c->grid;
+                }
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->grid = rc == HELM3D_OK ? best : keep;
+    CHK(ensure_work(c));
+    c->inner_graph_iters = -1;  // captured graphs hold the old launch shape
+    if (rc != HELM3D_OK) return rc;
+    if (kchunk) *kchunk = best.kchunk;
+    if (tile_rows) *tile_rows = best.ty;
+    if (prefetch) *prefetch = best.pf;
+    if (best_ms) *best_ms = best_t;
+    return HELM3D_OK;
+}
+
 int helm3d_apply(helm3d_ctx* c, int variant, int precision, const void* x, void* y)
 {
     CHK(check_model(c));

@@ -44,7 +44,7 @@ __device__ inline void block_sum(double (&v)[NV], double* red /* [KB/64][NV] */)
     }
 }
 
-// Contiguous range [lo, hi) of block b (even length, for paired points).
+// Contiguous range [lo, hi) of block b (even length).
 __device__ inline void range_even(long n, int b, int G, long& lo, long& hi)
 {
     const long per = ((n + G - 1) / G + 1) & ~1L;
@@ -79,14 +79,16 @@ __device__ __forceinline__ void dots_body(const cpx<R>* __restrict__ V, long ldv
     for (int q = 0; q < 2 * K; ++q) acc[q] = 0.0;
     long lo, hi;
     range_even(n, b, G, lo, hi);
-    long t = lo + 2 * threadIdx.x;
-    for (; t + 1 < hi; t += 2 * KB) {
-        const cpx<R> w0 = w[t], w1 = w[t + 1];
+    // two points per thread and iteration, t and t + KB: every load
+    // instruction reads one contiguous run of KB complex values
+    long t = lo + threadIdx.x;
+    for (; t + KB < hi; t += 2 * KB) {
+        const cpx<R> w0 = w[t], w1 = w[t + KB];
         cpx<R> v0[K], v1[K];
 #pragma unroll
         for (int c = 0; c < K; ++c) {
             v0[c] = V[c * ldv + t];
-            v1[c] = V[c * ldv + t + 1];
+            v1[c] = V[c * ldv + t + KB];
         }
 #pragma unroll
         for (int c = 0; c < K; ++c) {
@@ -129,17 +131,17 @@ __device__ __forceinline__ void update_body(const cpx<R>* __restrict__ V, long l
     };
     long lo, hi;
     range_even(n, b, G, lo, hi);
-    long t = lo + 2 * threadIdx.x;
-    for (; t + 1 < hi; t += 2 * KB) {
-        const cpx<R> w0 = w[t], w1 = w[t + 1];
+    long t = lo + threadIdx.x;
+    for (; t + KB < hi; t += 2 * KB) {
+        const cpx<R> w0 = w[t], w1 = w[t + KB];
         cpx<R> v0[K], v1[K];
 #pragma unroll
         for (int c = 0; c < K; ++c) {
             v0[c] = V[c * ldv + t];
-            v1[c] = V[c * ldv + t + 1];
+            v1[c] = V[c * ldv + t + KB];
         }
         w[t] = upd(w0, v0);
-        w[t + 1] = upd(w1, v1);
+        w[t + KB] = upd(w1, v1);
     }
     if (t < hi) {
         cpx<R> v0[K];

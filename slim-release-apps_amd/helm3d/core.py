@@ -241,6 +241,17 @@ class Helm3d:
         with torch.cuda.device(self.device):
             check(lib().helm3d_set_launch(self._ctx, int(kchunk), int(tile_rows), int(prefetch)))
 
+    def autotune(self, mode: int = 1, precision: str = "c128", reps: int = 10) -> dict:
+        """Pick the fastest stencil launch shape for this grid (helm3d_autotune)."""
+        prec = {"c128": _lib.HELM3D_C128, "c64": _lib.HELM3D_C64}[precision]
+        v = [ctypes.c_int(0) for _ in range(3)]
+        ms = ctypes.c_double(0)
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_autotune(self._ctx, int(mode), prec, int(reps), *[ctypes.byref(a) for a in v],
+                                        ctypes.byref(ms)))
+        return dict(kchunk=v[0].value, tile_rows=v[1].value, prefetch=v[2].value, ms=ms.value)
+
     def get_launch(self) -> dict:
         v = [ctypes.c_int(0) for _ in range(4)]
         with torch.cuda.device(self.device):

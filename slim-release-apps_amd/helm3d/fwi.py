@@ -80,7 +80,9 @@ class FWIOpts:
 
     solver: "fgmres" (FGMRES.m in complex128) or "mixed" (FP64 flexible GMRES
     with a complex64 BiCGStab preconditioner); tol is the relative residual of
-    every solve.  batch > 1 solves that many sources of a frequency together
+    every solve.  autotune picks the stencil launch shape per frequency
+    (one-at-a-time solves).
+    batch > 1 solves that many sources of a frequency together
     (helm3d_batch_solve: FGMRES_MT.m semantics for "fgmres", a fixed
     inner_iters complex64 BiCGStab iterations per outer step for "mixed")."""
     pml: Optional[object] = 10
@@ -98,6 +100,7 @@ class FWIOpts:
     inner_maxit: int = 200
     batch: int = 1
     inner_iters: int = 20
+    autotune: bool = False
     stats: list = field(default_factory=list, repr=False)
 
 
@@ -214,6 +217,9 @@ class FWIProblem:
         wn, dwn, _ = self.H.param2wavenum(v, self.n, p6, freq, m.unit, derivs=1)
         nt = tuple(self.n[a] + p6[2 * a] + p6[2 * a + 1] for a in range(3))
         self.H.set_model(nt, self.d, p6, wn)
+        if self.opts.autotune and self.opts.batch <= 1:
+            # launch shape of the stencil the one-at-a-time solves spend their time in
+            self.H.autotune(mode=1, precision="c64" if self.opts.solver == "mixed" else "c128", reps=5)
         Ps, Pr, PrT = self._geometry(npml)
         return dict(npml=npml, p6=p6, nt=nt, N=int(np.prod(nt)), wn=wn, dwn=dwn, Ps=Ps, Pr=Pr, PrT=PrT)
 

@@ -126,3 +126,22 @@ def test_zero_rhs_and_determinism():
         assert ia[1].converged and ia[1].relres == 0.0 and not xa[1].any()
         assert torch.equal(xa[0], xb[0])
         assert ia[0].iters == ib[0].iters
+
+
+def test_batch_autotune_keeps_results():
+    import torch
+    from helm3d import BatchSolver
+    cfg, wn, H = _op((32, 32, 24), 8, 5.0, 18)
+    b = torch.from_numpy(_rhs(cfg, cfg.sources[0])).cuda()
+    S = BatchSolver("cuda:0")
+    x0, i0 = S.solve([H, H], [b, 2 * b], method="mixed", tol=1e-6, inner_maxit=8)
+    r = S.autotune([H, H], [b, 2 * b], inner_maxit=8, divisors=(1, 2))
+    print("batch autotune", r, flush=True)
+    L = H.get_launch()
+    assert (L["tile_rows"], L["prefetch"]) == (r["tile_rows"], r["prefetch"])
+    x1, i1 = S.solve([H, H], [b, 2 * b], method="mixed", tol=1e-6, inner_maxit=8)
+    # the dot products reduced in the stencil epilogues follow the block
+    # decomposition, so another shape changes rounding only
+    for a, c, ia, ic in zip(x0, x1, i0, i1):
+        assert ic.converged and abs(ia.iters - ic.iters) <= 2
+        assert float((a - c).norm() / a.norm()) < 1e-4

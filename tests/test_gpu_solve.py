@@ -86,8 +86,13 @@ def test_bicgstab_fp64_matches_oracle():
     x, info = H.solve_bicgstab(torch.from_numpy(b).to(dev), tol=1e-6)
     print("bicgstab gpu matvecs", info.iters, "oracle iters", it_o, info.relres)
     assert info.converged
-    assert abs(info.iters // 2 - it_o) <= 0.05 * it_o + 2
-    r = b - O.hmvp("forw", wn, h3, cfg.nt, p6, x.cpu().numpy())
+    # BiCGStab on this indefinite operator amplifies rounding differences of
+    # the operator (1e-16 relative) into different iterate paths, so the step
+    # counts agree only roughly; the solutions must agree to the tolerance.
+    assert 0.75 * it_o <= info.iters // 2 <= 1.33 * it_o + 2
+    xg = x.cpu().numpy()
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-4
+    r = b - O.hmvp("forw", wn, h3, cfg.nt, p6, xg)
     assert abs(np.linalg.norm(r) / np.linalg.norm(b) - info.relres) < 1e-9
     assert info.relres < 1e-5
 

@@ -148,3 +148,26 @@ def test_c2_full_size_matvec_parity():
         y = H.apply(xt, adjoint=adj).cpu().numpy()
         yo = O.hmvp(v, wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), x)
         assert relerr(y, yo) <= 1e-12, v
+
+
+def test_autotune_keeps_results():
+    """The autotuner picks a valid shape; products are unchanged by it."""
+    import torch
+    from helm3d import Helm3d
+    n, npml, h = CASES[5]
+    wn, x = _inputs(n, 77, False)
+    dev = torch.device("cuda:0")
+    H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).to(dev))
+    xt = torch.from_numpy(x).to(dev)
+    y0 = H.apply(xt).clone()
+    a0 = H.apply(xt, adjoint=True).clone()
+    s0 = H.apply(xt.to(torch.complex64)).clone()
+    for mode, prec in ((1, "c128"), (2, "c128"), (1, "c64")):
+        r = H.autotune(mode=mode, precision=prec, reps=3)
+        print("autotune", mode, prec, r, flush=True)
+        assert r["tile_rows"] in (4, 8) and r["prefetch"] in (1, 2) and 1 <= r["kchunk"] <= n[2] and r["ms"] > 0
+        L = H.get_launch()
+        assert (L["kchunk"], L["tile_rows"], L["prefetch"]) == (r["kchunk"], r["tile_rows"], r["prefetch"])
+        assert torch.equal(H.apply(xt), y0)
+        assert torch.equal(H.apply(xt, adjoint=True), a0)
+        assert torch.equal(H.apply(xt.to(torch.complex64)), s0)
