@@ -113,16 +113,87 @@ def fwi_wavelet(f, t0: float, f0: float):
     return w
 
 
+class FWIFreqData:
+    """Frequency-domain data with its acquisition metadata (FWIFreqData.m:27-48,
+    104-113, numeric-data form): nrec x (nsrc nfreq) with frequencies outer,
+    as PDEfunc indexes it (column k*nsrc + j for frequency k, source j)."""
+
+    def __init__(self, D, model: "FWIModel"):
+        self.freqgrid = list(model.freq)
+        self.model = model
+        D = D if isinstance(D, torch.Tensor) else torch.as_tensor(np.asarray(D))
+        nrec = model.nrec
+        if D.numel() % nrec:
+            raise ValueError("data size is not a multiple of the receiver count")
+        if D.dim() == 2 and D.shape[0] == nrec:
+            self.localData = D
+        else:
+            # reshape(D, nrec, []) of a column-major (MATLAB) array
+            self.localData = D.reshape(-1).reshape(D.numel() // nrec, nrec).T
+
+    def getData(self, index=None):
+        return self.localData if index is None else self.localData[:, index]
+
+    def setData(self, index, data):
+        self.localData[:, index] = torch.as_tensor(data, dtype=self.localData.dtype,
+                                                   device=self.localData.device).reshape(self.localData.shape[0], -1)
+
+    def getSourceGrid(self):
+        m = self.model
+        return dict(src=(m.xsrc, m.ysrc, m.zsrc), rec=(m.xrec, m.yrec, m.zrec))
+
+
 def grid_coords(o: float, d: float, n: int) -> np.ndarray:
     return o + d * np.arange(n)
 
 
+def lagrange_cubic(x1, x2) -> np.ndarray:
+    """Weights (len(x2) x len(x1)) of opLInterp1D(x1, x2) (opLInterp1D.m:114-153):
+    4-point Lagrange interpolation from the grid x1 to the points x2, with the
+    bracketing window walked from point to point as the reference does; points
+    outside [x1(1), x1(end)] get a zero row.  Needs at least 4 points in x1."""
+    x1 = np.asarray(x1, dtype=np.float64).reshape(-1)
+    x2 = np.asarray(x2, dtype=np.float64).reshape(-1)
+    n1 = x1.size
+    W = np.zeros((x2.size, n1))
+    ik = np.nonzero((x2 >= x1[0]) & (x2 <= x1[-1]))[0]
+    if ik.size == 0:
+        return W
+    if n1 < 4:
+        raise ValueError("cubic interpolation needs at least 4 grid points")
+    a, b, c, d = 0, 1, 2, 3  # 0-based window
+    for k in ik:
+        x = x2[k]
+        if x < x1[b]:
+            while x < x1[b] and b - 1 > 0:
+                b -= 1
+            a, c = b - 1, b + 1
+            d = c + 1
+        elif x > x1[c]:
+            while x > x1[c] and c + 1 < n1 - 1:
+                c += 1
+            a, b, d = c - 2, c - 1, c + 1
+        idx = (a, b, c, d)
+        for q in range(4):
+            num = 1.0
+            den = 1.0
+            for r in range(4):
+                if r != q:
+                    num *= x - x1[idx[r]]
+                    den *= x1[idx[q]] - x1[idx[r]]
+            W[k, idx[q]] += num / den
+    return W
+
+
 def interp_1d(xin, xout, kind: str = "sinc") -> np.ndarray:
-    """Weights (len(xout) x len(xin)) of opSincInterp(xin, xout, 4): the
-    Kaiser-windowed sinc of (xout - xin)/dx with dx the xout spacing
-    (opSincInterp.m:26-51, kaiser_window.m:17-19)."""
+    """Weights (len(xout) x len(xin)) of the 1D interpolation of opInterp.m:41-47:
+    "sinc": opSincInterp(xin, xout, 4), the Kaiser-windowed sinc of
+    (xout - xin)/dx with dx the xout spacing (opSincInterp.m:26-51,
+    kaiser_window.m:17-19); "cubic": opLInterp1D(xin, xout)."""
+    if kind == "cubic":
+        return lagrange_cubic(xin, xout)
     if kind != "sinc":
-        raise NotImplementedError(f"interpolation '{kind}' (only 'sinc' on this path)")
+        raise ValueError("Unrecognized type")
     xin = np.asarray(xin, dtype=np.float64).reshape(-1)
     xout = np.asarray(xout, dtype=np.float64).reshape(-1)
     if xout.size > 1:
@@ -340,6 +411,8 @@ class FWIProblem:
         return t
 
     def _data(self, Dobs) -> torch.Tensor:
+        if isinstance(Dobs, FWIFreqData):
+            Dobs = Dobs.getData()
         D = Dobs if isinstance(Dobs, torch.Tensor) else torch.from_numpy(np.asarray(Dobs))
         D = D.to(device=self.device, dtype=torch.complex128)
         npairs = len(self.model.freq) * self.Q.shape[1]

@@ -153,3 +153,40 @@ def test_jacobian_adjoint_pair_and_gauss_newton():
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
     hgn = P.hess_gn(m0, dm)
     assert np.linalg.norm(hgn - P.jacob_adj(m0, Jdm)) <= 1e-10 * np.linalg.norm(hgn)
+
+
+def test_cubic_interpolation_matches_reference_self_test():
+    """opLInterp1D.m:58-94 checks its own operator: exact on cubics (zero
+    outside the grid) and a consistent adjoint; on-grid points select."""
+    from helm3d.fwi import lagrange_cubic, interp_1d
+    rng = np.random.default_rng(2)
+    x1 = np.sort(rng.uniform(0, 100, 12))
+    x2 = np.sort(rng.uniform(-10, 110, 40))
+    A = lagrange_cubic(x1, x2)
+    f1 = x1 ** 3
+    f2 = x2 ** 3
+    f2[(x2 < x1[0]) | (x2 > x1[-1])] = 0
+    assert np.linalg.norm(f2 - A @ f1) / np.linalg.norm(f2) < 1e-10
+    g1 = rng.standard_normal(x1.size)
+    g2 = rng.standard_normal(x2.size)
+    assert abs(1 - ((A @ g1) @ g2) / (g1 @ (A.T @ g2))) < 1e-10
+    # on-grid points reduce to a selection
+    S = interp_1d(x1, x1[[0, 3, 7, 11]], "cubic")
+    assert np.allclose(S, np.eye(12)[[0, 3, 7, 11]], atol=1e-12)
+    with pytest.raises(ValueError):
+        interp_1d(x1, x2, "linear-ish")
+
+
+def test_freq_data_container():
+    import torch
+    from helm3d.fwi import FWIFreqData, FWIModel
+    model = FWIModel(o=(0, 0, 0), d=(10, 10, 10), n=(8, 8, 8), freq=[3.0, 5.0], xsrc=[10.0, 20.0], ysrc=[10.0],
+                     zsrc=[10.0], xrec=[0.0, 10.0, 20.0], yrec=[0.0], zrec=[0.0])
+    D = np.arange(3 * 2 * 2) + 1j * np.arange(12)
+    F = FWIFreqData(D, model)
+    assert F.localData.shape == (3, 4)
+    assert np.array_equal(F.getData(2).numpy(), D[6:9])  # column-major reshape(D, nrec, [])
+    assert np.array_equal(FWIFreqData(D.reshape(3, 4, order="F"), model).getData(2).numpy(), D[6:9])
+    F.setData([1], np.zeros(3))
+    assert not F.getData(1).any()
+    assert F.freqgrid == [3.0, 5.0]
