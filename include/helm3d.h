@@ -38,6 +38,7 @@ extern "C" {
 #define HELM3D_ERR_NOMODEL 5  /* context has no model set                                                */
 #define HELM3D_ERR_NOMEM   6  /* device memory exhausted                                                 */
 #define HELM3D_ERR_NOTCONV 7  /* solve stopped at maxit above tolerance; x holds the last iterate        */
+#define HELM3D_ERR_BSIZE   8  /* b does not have prod(n) rows (Helm3dkaczswp_mex:Bsize)  */
 
 /* ---- operator variants -------------------------------------------------- *
  * The six MEX binaries of ibin/install_MEX:88-105 map to
@@ -237,6 +238,35 @@ int helm3d_batch_set_stream(helm3d_batch *batch, void *stream);
 int helm3d_batch_solve(helm3d_batch *batch, int n, helm3d_ctx *const *ctx, int mode,
                        const void *const *b, void *const *x, int method,
                        const helm3d_solve_opts *opts, helm3d_solve_info *info);
+
+/* ---- Kaczmarz sweeps and CGMN ----------------------------------------------- */
+
+/* nsweeps double Kaczmarz sweeps of x (in place) for A x = b, A = H (mode 1)
+ * or H^H (otherwise), relaxation omega: the row projections of
+ * Helm3dkaczswp_mex.c:60-117, forward then backward, with the rows ordered
+ * by 27 colours (i mod 3, j mod 3, k mod 3) instead of lexicographically so
+ * that each colour is one parallel launch (the iterates differ from the
+ * lexicographic sweep; its fixed point does not).  b may be NULL (zero). */
+int helm3d_kacz_sweep(helm3d_ctx *ctx, int mode, double omega, int nsweeps, void *x_dev,
+                      const void *b_dev);
+
+/* Drop-in of y = Helm3d_kacz_sweep{,_wnc,_adj}_mex(wn,h,n,npml,w,x,b)
+ * (helm3d_operto27_kaczswp.m:2-12, Helm3dkaczswp_mex.c:123-197) on host
+ * split-complex arrays, one right-hand side: variant HELM3D_FORW, HELM3D_ADJ,
+ * or either with HELM3D_WNC (complex wn honoured); xi, bi may be NULL.
+ * Errors: HELM3D_ERR_NSIZE, HELM3D_ERR_XSIZE, HELM3D_ERR_BSIZE as the MEX. */
+int helm3d_kaczswp(int variant, const double *wnr, const double *wni, const double *h,
+                   const double *n, long long n_len, const double *npml, double omega,
+                   const double *xr, const double *xi, long long x_rows, const double *br,
+                   const double *bi, long long b_rows, double *yr, double *yi);
+
+/* CGMN (CGMN.m) on the Kaczmarz system of linearsolve.m:52-58 with
+ * omega = 1.5 and one double sweep: b' = SWP(0, b), A' y = y - SWP(y, 0), CG
+ * from x = 0 until ||r'|| / ||b'|| <= tol or maxit iterations.  info->iters:
+ * CG iterations; info->quasi_relres: the preconditioned residual;
+ * info->relres: the true ||b - A x|| / ||b||. */
+int helm3d_solve_cgmn(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
+                      const helm3d_solve_opts *opts, helm3d_solve_info *info);
 
 /* ---- FWI gradient ------------------------------------------------------- */
 

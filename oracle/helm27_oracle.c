@@ -26,6 +26,9 @@
  * Complex vectors cross this interface as split real/imaginary arrays, as at
  * the MEX boundary (Helm3dmvp_mex.c:67-98).
  *
+ * Kaczmarz sweeps (Helm3dkaczswp_mex.c:60-117) are restated at the end of the
+ * file; pinned by tests/test_oracle_kaczmarz.py.
+ *
  * Pinned by: tests/test_oracle_stencil.py compares every variant with the
  * reference header compiled in place (oracle/_ref) on seeded inputs, checks
  * <Hx,y> = <x,H^H y> for (fwd, adj) and (deriv, adj-deriv), the DERIV linearity
@@ -301,4 +304,154 @@ void oracle_hconst(const double *h, double *out8)
     hconst k = make_hconst(h);
     out8[0] = k.wn_coef; out8[1] = k.c[0]; out8[2] = k.c[1]; out8[3] = k.c[2];
     out8[4] = k.xy; out8[5] = k.xz; out8[6] = k.yz; out8[7] = k.w3a2;
+}
+
+/* ------------------------------------------------------------------------
+ * Kaczmarz row projections (Helm3dkaczswp_mex.c:60-117).
+ *
+ * Row p of H (adj = 0) or of H^H (adj = 1): the same coefficients as the
+ * matvec above, listed for the in-grid neighbours in the order dz, dy, dx =
+ * -1..1 (x fastest).  A projection with relaxation omega:
+ *   d = omega (b_p - a.x) / |a|^2 ;  x_q += d conj(a_q)   (:96-117)
+ * A double sweep visits every row forward and then backward
+ * (xyzloop_updown, Helm3d_27pt.h:63-108).  Two orders:
+ *   lex: k, j, i ascending, then descending (the reference's sweep);
+ *   mc:  27 colours c = (i mod 3) + 3 (j mod 3) + 9 (k mod 3) ascending, then
+ *        descending; rows of one colour have disjoint 3x3x3 supports, so
+ *        their order inside the colour does not matter (the GPU form).
+ * wn complex (wnc) is honoured for both orders.
+ * ------------------------------------------------------------------------ */
+
+static int row_of(int adj, int wnc, const double *wnr, const double *wni, const hconst *hc,
+                  const axis_tab ax[3], const int n[3], const int p[3], cplx coef[27], long idx[27])
+{
+    const long sy = n[0], sz = (long)n[0] * n[1];
+    const long ip = p[0] + p[1] * sy + p[2] * sz;
+    int cnt = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int d[3] = {dx, dy, dz};
+                int q[3], inside = 1;
+                for (int a = 0; a < 3; a++) {
+                    q[a] = p[a] + d[a];
+                    if (q[a] < 0 || q[a] >= n[a]) inside = 0;
+                }
+                if (!inside) continue;
+                const long iq = q[0] + q[1] * sy + q[2] * sz;
+                const int nnz = (dx != 0) + (dy != 0) + (dz != 0);
+                cplx c, lo[3], hi[3];
+                if (!adj) {
+                    cplx w = wnc ? wnr[iq] + wni[iq] * I : wnr[iq];
+                    for (int a = 0; a < 3; a++) {
+                        lo[a] = ax[a].lo[p[a] + 1];
+                        hi[a] = ax[a].hi[p[a] + 1];
+                    }
+                    c = mass_mu(hc, nnz) * w + stencil_K(hc, d, lo, hi);
+                } else {
+                    cplx w = wnc ? wnr[ip] + wni[ip] * I : wnr[ip];
+                    int md[3] = {-dx, -dy, -dz};
+                    for (int a = 0; a < 3; a++) {
+                        lo[a] = ax[a].lo[q[a] + 1];
+                        hi[a] = ax[a].hi[q[a] + 1];
+                    }
+                    c = conj(mass_mu(hc, nnz) * w + stencil_K(hc, md, lo, hi));
+                }
+                coef[cnt] = c;
+                idx[cnt] = iq;
+                cnt++;
+            }
+    return cnt;
+}
+
+static void project(int adj, int wnc, const double *wnr, const double *wni, const hconst *hc,
+                    const axis_tab ax[3], const int n[3], const int p[3], double omega, double *xr,
+                    double *xi, const double *br, const double *bi)
+{
+    cplx coef[27];
+    long idx[27];
+    const int cnt = row_of(adj, wnc, wnr, wni, hc, ax, n, p, coef, idx);
+    const long ip = p[0] + p[1] * (long)n[0] + p[2] * (long)n[0] * n[1];
+    cplx atx = 0;
+    double nrm = 0;
+    for (int t = 0; t < cnt; t++) {
+        atx += coef[t] * (xr[idx[t]] + xi[idx[t]] * I);
+        nrm += creal(conj(coef[t]) * coef[t]);
+    }
+    cplx d = omega * ((br[ip] + (bi ? bi[ip] : 0.0) * I) - atx) / nrm;
+    for (int t = 0; t < cnt; t++) {
+        cplx u = d * conj(coef[t]);
+        xr[idx[t]] += creal(u);
+        xi[idx[t]] += cimag(u);
+    }
+}
+
+/* variant: bit 0 adjoint, bit 2 complex wn.  x (xr, xi) is swept in place. */
+static void kacz_sweep(int order_mc, int variant, const double *wnr, const double *wni, const double *h,
+                       const double *nd, const double *npml, double omega, double *xr, double *xi,
+                       const double *br, const double *bi)
+{
+    const int adj = variant & 1, wnc = (variant & 4) != 0;
+    const int n[3] = {(int)nd[0], (int)nd[1], (int)nd[2]};
+    const hconst hc = make_hconst(h);
+    axis_tab ax[3];
+    ax[0] = make_axis(n[0], (int)npml[0], (int)npml[1]);
+    ax[1] = make_axis(n[1], (int)npml[2], (int)npml[3]);
+    ax[2] = make_axis(n[2], (int)npml[4], (int)npml[5]);
+    for (int pass = 0; pass < 2; pass++) {
+        if (!order_mc) {
+            for (long t = 0; t < (long)n[0] * n[1] * n[2]; t++) {
+                const long s = pass == 0 ? t : (long)n[0] * n[1] * n[2] - 1 - t;
+                const int p[3] = {(int)(s % n[0]), (int)((s / n[0]) % n[1]), (int)(s / ((long)n[0] * n[1]))};
+                project(adj, wnc, wnr, wni, &hc, ax, n, p, omega, xr, xi, br, bi);
+            }
+        } else {
+            for (int cc = 0; cc < 27; cc++) {
+                const int c = pass == 0 ? cc : 26 - cc;
+                const int ci = c % 3, cj = (c / 3) % 3, ck = c / 9;
+                for (int k = ck; k < n[2]; k += 3)
+                    for (int j = cj; j < n[1]; j += 3)
+                        for (int i = ci; i < n[0]; i += 3) {
+                            const int p[3] = {i, j, k};
+                            project(adj, wnc, wnr, wni, &hc, ax, n, p, omega, xr, xi, br, bi);
+                        }
+            }
+        }
+    }
+    for (int a = 0; a < 3; a++) free_axis(&ax[a]);
+}
+
+void oracle_kacz_sweep_lex(int variant, const double *wnr, const double *wni, const double *h, const double *n,
+                           const double *npml, double omega, double *xr, double *xi, const double *br,
+                           const double *bi)
+{
+    kacz_sweep(0, variant, wnr, wni, h, n, npml, omega, xr, xi, br, bi);
+}
+
+void oracle_kacz_sweep_mc(int variant, const double *wnr, const double *wni, const double *h, const double *n,
+                          const double *npml, double omega, double *xr, double *xi, const double *br,
+                          const double *bi)
+{
+    kacz_sweep(1, variant, wnr, wni, h, n, npml, omega, xr, xi, br, bi);
+}
+
+/* Row p of H or H^H as (count, coefficients interleaved re/im, flat indices). */
+int oracle_row(int variant, const double *wnr, const double *wni, const double *h, const double *nd,
+               const double *npml, int i, int j, int k, double *coef_out, long *idx_out)
+{
+    const int n[3] = {(int)nd[0], (int)nd[1], (int)nd[2]};
+    const hconst hc = make_hconst(h);
+    axis_tab ax[3];
+    ax[0] = make_axis(n[0], (int)npml[0], (int)npml[1]);
+    ax[1] = make_axis(n[1], (int)npml[2], (int)npml[3]);
+    ax[2] = make_axis(n[2], (int)npml[4], (int)npml[5]);
+    cplx coef[27];
+    const int p[3] = {i, j, k};
+    const int cnt = row_of(variant & 1, (variant & 4) != 0, wnr, wni, &hc, ax, n, p, coef, idx_out);
+    for (int t = 0; t < cnt; t++) {
+        coef_out[2 * t] = creal(coef[t]);
+        coef_out[2 * t + 1] = cimag(coef[t]);
+    }
+    for (int a = 0; a < 3; a++) free_axis(&ax[a]);
+    return cnt;
 }

@@ -17,9 +17,12 @@ Contents
                             opExtension.m:89-90 (adjoint of zero padding)
       - ``fwi_wavelet``     tools/algorithms/CommonFreqModeling/fwi_wavelet.m:24-28
       - ``ls_misfit``       tools/algorithms/CommonFreqModeling/LS_misfit.m:21-26
-      - ``fwi_objective``   PDEfunc.m:324-394 (func = OBJ) with T_adj of
-                            discrete_helmholtz.m:288 and the source scaling of
-                            PDEfunc.m:328-339.
+      - ``kacz_sweep``      Kaczmarz double sweeps (Helm3dkaczswp_mex.c:60-117)
+                            in the reference's lexicographic order and in the
+                            27-colour order (C restatement in helm27_oracle.c)
+      - ``cgmn``            CGMN.m with the Kaczmarz preconditioned system of
+                            linearsolve.m:52-58
+    (the PDEfunc restatement lives in fwi_oracle.py).
 
 Parity status of each function is stated in its docstring; the C parts are
 pinned against ``oracle/_ref`` (see ``tests/test_oracle_stencil.py``).
@@ -67,6 +70,13 @@ def lib():
                                     ctypes.c_int, ctypes.c_long, ctypes.c_long,
                                     _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
                                     _dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long), _dp]
+        for nm in ("oracle_kacz_sweep_lex", "oracle_kacz_sweep_mc"):
+            f = getattr(L, nm)
+            f.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp]
+            f.restype = None
+        L.oracle_row.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 _dp, ctypes.POINTER(ctypes.c_long)]
+        L.oracle_row.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -260,3 +270,68 @@ def bicgstab(variant: str, wn, h, n, npml, b, tol=1e-6, maxit=10000, nthreads: i
         rho = rho_new
         p = r + beta * (p - omega * v)
     return x, it, np.array(hist)
+
+
+# ---------------------------------------------------------------------------
+# Kaczmarz sweeps and CGMN
+# ---------------------------------------------------------------------------
+
+_KVAR = {"forw": 0, "adj": 1, "forw_wnc": 4, "adj_wnc": 5}
+
+
+def kacz_sweep(variant: str, wn, h, n, npml, omega, x, b, order: str = "mc"):
+    """One double Kaczmarz sweep (forward then backward) of x for A x = b with
+    A = H (forw*) or H^H (adj*): Helm3dkaczswp_mex.c:60-117 in the reference's
+    lexicographic order (order="lex") or the 27-colour order ("mc").
+    Returns the swept copy of x."""
+    wnr, wni, h, n, npml, xr, xi, N = _prep(wn, h, n, npml, x)
+    xr = xr.copy()
+    xi = (xi.copy() if xi is not None else np.zeros(N))
+    b = np.asarray(b).reshape(-1)
+    br = np.ascontiguousarray(b.real, dtype=np.float64)
+    bi = np.ascontiguousarray(b.imag, dtype=np.float64)
+    if wni is None:
+        wni = np.zeros(N)
+    f = lib().oracle_kacz_sweep_mc if order == "mc" else lib().oracle_kacz_sweep_lex
+    f(_KVAR[variant], _p(wnr), _p(wni), _p(h), _p(n), _p(npml), float(omega), _p(xr), _p(xi), _p(br), _p(bi))
+    return xr + 1j * xi
+
+
+def row(variant: str, wn, h, n, npml, i, j, k):
+    """Row (i,j,k) of H or H^H: (coefficients, flat neighbour indices)."""
+    wnr, wni, h, n, npml, _, _, N = _prep(wn, h, n, npml, np.zeros(int(np.prod(n))))
+    if wni is None:
+        wni = np.zeros(N)
+    c = np.zeros(54)
+    idx = (ctypes.c_long * 27)()
+    cnt = lib().oracle_row(_KVAR[variant], _p(wnr), _p(wni), _p(h), _p(n), _p(npml), int(i), int(j), int(k),
+                           _p(c), idx)
+    return c[:2 * cnt:2] + 1j * c[1:2 * cnt:2], np.array(idx[:cnt])
+
+
+def cgmn(sweep, b, tol=1e-6, maxit=10000):
+    """CGMN.m on the Kaczmarz-preconditioned system of linearsolve.m:52-58:
+    b' = SWP(0, b), A' y = y - SWP(y, 0), CG from x = 0.  sweep(x, rhs) is one
+    double sweep.  Returns (x, iterations, relative residual history)."""
+    zb = np.zeros_like(b)
+    bp = sweep(np.zeros_like(b), b)
+    A = lambda y: y - sweep(y, zb)  # noqa: E731
+    x = np.zeros_like(b)
+    r = bp.copy()
+    p = r.copy()
+    normb = np.linalg.norm(bp)
+    normr = np.linalg.norm(r)
+    hst = [normr / normb]
+    k = 0
+    while True:
+        Ap = A(p)
+        alpha = normr ** 2 / np.vdot(p, Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        normr_old, normr = normr, np.linalg.norm(r)
+        hst.append(normr / normb)
+        k += 1
+        if k >= maxit or hst[-1] <= tol:
+            break
+        p = r + (normr / normr_old) ** 2 * p
+    return x, k, np.array(hst)

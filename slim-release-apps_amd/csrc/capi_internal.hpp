@@ -198,4 +198,23 @@ inline int bind(helm3d_ctx* c)
 // every slot starts on a 4 KiB boundary.
 inline long slot_ld(long N) { return (N + 255) / 256 * 256; }
 
+// Context work buffers: reduction partials, reduced values (+ pinned host
+// mirror) and the stencil's write-only sink.
+inline int ensure_work(helm3d_ctx* c)
+{
+    const size_t nblk = std::max<size_t>(c->grid.blocks, kMaxPartials);
+    CHK(ensure_t(&c->part, &c->part_cap, nblk * 2 * kMaxBasis));
+    CHK(ensure_t(&c->small, &c->small_cap, 2 * kMaxBasis + 8));
+    CHK(ensure(&c->sink, &c->sink_cap, size_t(kSinkSlots) * sizeof(cpx<double>)));
+    if (!c->small_host)
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->small_host), (2 * kMaxBasis + 8) * sizeof(double)));
+    return HELM3D_OK;
+}
+
+template <typename R>
+inline int ensure_vecs(Vecs<R>& v, long N, int slots)
+{
+    return ensure_t(&v.p, &v.cap, size_t(slot_ld(N)) * slots);
+}
+
 }  // namespace helm3d

@@ -79,21 +79,6 @@ template <>
 Vecs<float>& basis_of<float>(helm3d_ctx* c) { return c->V32; }
 
 
-int ensure_work(helm3d_ctx* c)
-{
-    const size_t nblk = std::max<size_t>(c->grid.blocks, kMaxPartials);
-    CHK(ensure_t(&c->part, &c->part_cap, nblk * 2 * kMaxBasis));
-    CHK(ensure_t(&c->small, &c->small_cap, 2 * kMaxBasis + 8));
-    CHK(ensure(&c->sink, &c->sink_cap, size_t(kSinkSlots) * sizeof(cpx<double>)));
-    if (!c->small_host) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->small_host), (2 * kMaxBasis + 8) * sizeof(double)));
-    return HELM3D_OK;
-}
-
-template <typename R>
-int ensure_vecs(Vecs<R>& v, long N, int slots)
-{
-    return ensure_t(&v.p, &v.cap, size_t(slot_ld(N)) * slots);
-}
 
 // Stencil launch on the context's operator; y = alpha*Hx + beta*b.
 template <typename R>

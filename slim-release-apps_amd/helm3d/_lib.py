@@ -24,6 +24,7 @@ HELM3D_ERR_HIP = 4
 HELM3D_ERR_NOMODEL = 5
 HELM3D_ERR_NOMEM = 6
 HELM3D_ERR_NOTCONV = 7
+HELM3D_ERR_BSIZE = 8
 
 HELM3D_FORW = 0
 HELM3D_ADJ = 1
@@ -49,6 +50,12 @@ MEX_ERR_IDS = {
     HELM3D_ERR_NSIZE: "SLIM_release_apps:tools:algorithms:ThreeDFreqModeling:Helm3dmvp_mex:Nsize",
     HELM3D_ERR_XSIZE: "SLIM_release_apps:tools:algorithms:ThreeDFreqModeling:Helm3dkaczswp_mex:Xsize",
 }
+# identifiers of the sweep gateway (Helm3dkaczswp_mex.c:150,159,164)
+MEX_SWEEP_ERR_IDS = {
+    HELM3D_ERR_NSIZE: "SLIM_release_apps:tools:algorithms:ThreeDFreqModeling:Helm3dkaczswp_mex:Nsize",
+    HELM3D_ERR_XSIZE: "SLIM_release_apps:tools:algorithms:ThreeDFreqModeling:Helm3dkaczswp_mex:Xsize",
+    HELM3D_ERR_BSIZE: "SLIM_release_apps:tools:algorithms:ThreeDFreqModeling:Helm3dkaczswp_mex:Bsize",
+}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -71,10 +78,10 @@ class SolveInfo(ctypes.Structure):
 
 
 class Helm3dError(RuntimeError):
-    def __init__(self, code: int, msg: str):
+    def __init__(self, code: int, msg: str, ids=None):
         super().__init__(f"helm3d error {code}: {msg}")
         self.code = code
-        self.identifier = MEX_ERR_IDS.get(code, "")
+        self.identifier = (ids if ids is not None else MEX_ERR_IDS).get(code, "")
 
 
 _lib = None
@@ -111,6 +118,11 @@ _SIGS = [
     ("helm3d_batch_solve", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int,
                                           ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
                                           ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    ("helm3d_kacz_sweep", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
+    ("helm3d_kaczswp", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_longlong, _dp, ctypes.c_double,
+                                      _dp, _dp, ctypes.c_longlong, _dp, _dp, ctypes.c_longlong, _dp, _dp]),
+    ("helm3d_solve_cgmn", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
+                                         ctypes.POINTER(SolveInfo)]),
     ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
     ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
@@ -142,7 +154,7 @@ def lib():
     return _lib
 
 
-def check(code: int, allow=()):
+def check(code: int, allow=(), ids=None):
     if code != HELM3D_OK and code not in allow:
-        raise Helm3dError(code, lib().helm3d_last_error().decode())
+        raise Helm3dError(code, lib().helm3d_last_error().decode(), ids)
     return code

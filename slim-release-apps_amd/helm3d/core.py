@@ -230,6 +230,36 @@ class Helm3d:
         return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
                             bool(info.converged))
 
+    def kacz_sweep(self, x: torch.Tensor, b: Optional[torch.Tensor] = None, mode: int = 1, omega: float = 1.5,
+                   nsweeps: int = 1) -> torch.Tensor:
+        """nsweeps double Kaczmarz sweeps (27-colour order) of a copy of x for
+        A x = b, A = H (mode 1) or H^H (Helm3dkaczswp_mex.c:60-117)."""
+        y = self._vec(x, torch.complex128).clone()
+        bb = self._vec(b, torch.complex128) if b is not None else None
+        with torch.cuda.device(self.device):
+            self._bind()
+            check(lib().helm3d_kacz_sweep(self._ctx, int(mode), float(omega), int(nsweeps),
+                                          ctypes.c_void_p(_dptr(y)),
+                                          ctypes.c_void_p(_dptr(bb)) if bb is not None else None))
+        return y
+
+    def solve_cgmn(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxit: int = 10000,
+                   raise_on_notconv: bool = False):
+        """CGMN.m on the Kaczmarz system (linearsolve.m:52-58, omega 1.5):
+        iters counts CG iterations, quasi_relres is the preconditioned residual,
+        relres the true one."""
+        b = self._vec(b, torch.complex128)
+        x = torch.zeros_like(b)
+        o = self._opts(tol, 20, maxit, False)
+        info = _lib.SolveInfo()
+        with torch.cuda.device(self.device):
+            self._bind()
+            code = lib().helm3d_solve_cgmn(self._ctx, int(mode), ctypes.c_void_p(_dptr(b)),
+                                           ctypes.c_void_p(_dptr(x)), ctypes.byref(o), ctypes.byref(info))
+        check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
+        return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
+                            bool(info.converged), inner_iters=int(info.inner_iters))
+
     def set_graphs(self, enable: bool):
         """Replay fixed kernel sequences from captured HIP graphs (default on)."""
         with torch.cuda.device(self.device):
@@ -427,4 +457,37 @@ def helm3d_mvp_host(variant, wn, h, n, npml, x, nthreads: int = 1) -> np.ndarray
     code = lib().helm3d_mvp(v, P(wnr), P(wni), P(hh), P(n_arr), n_arr.size, P(pp), P(yr), P(yi), P(xr), P(xi),
                             x_rows, int(nthreads))
     check(code)
+    return yr + 1j * yi
+
+
+_SWEEP_VARIANTS = {"forw": _lib.HELM3D_FORW, "adj": _lib.HELM3D_ADJ,
+                   "forw_wnc": _lib.HELM3D_FORW | _lib.HELM3D_WNC, "adj_wnc": _lib.HELM3D_ADJ | _lib.HELM3D_WNC}
+
+
+def helm3d_kaczswp_host(variant, wn, h, n, npml, omega, x, b) -> np.ndarray:
+    """y = Helm3d_kacz_sweep{,_wnc,_adj}_mex(wn, h, n, npml, w, x, b) on host
+    arrays, one right-hand side (27-colour double sweep on the GPU)."""
+    v = _SWEEP_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+    wn = np.asarray(wn).reshape(-1)
+    n_arr = np.ascontiguousarray(np.asarray(n, dtype=np.float64).reshape(-1))
+    x = np.asarray(x)
+    b = np.asarray(b)
+    x_rows = x.shape[0] if x.ndim >= 1 else 1
+    b_rows = b.shape[0] if b.ndim >= 1 else 1
+    x = x.reshape(x_rows, -1)[:, 0] if x.ndim > 1 else x.reshape(-1)
+    b = b.reshape(b_rows, -1)[:, 0] if b.ndim > 1 else b.reshape(-1)
+    N = x.size
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    wnr = f(wn.real)
+    wni = f(wn.imag) if np.iscomplexobj(wn) else None
+    xr, xi = f(x.real), (f(x.imag) if np.iscomplexobj(x) else None)
+    br, bi = f(b.real), (f(b.imag) if np.iscomplexobj(b) else None)
+    hh = f(np.asarray(h).reshape(3))
+    pp = f(np.asarray(npml_flat(npml)))
+    yr = np.zeros(N)
+    yi = np.zeros(N)
+    P = lambda a: a.ctypes.data_as(_lib._dp) if a is not None else None  # noqa: E731
+    code = lib().helm3d_kaczswp(v, P(wnr), P(wni), P(hh), P(n_arr), n_arr.size, P(pp), float(omega), P(xr), P(xi),
+                                x_rows, P(br), P(bi), b_rows, P(yr), P(yi))
+    check(code, ids=_lib.MEX_SWEEP_ERR_IDS)
     return yr + 1j * yi
