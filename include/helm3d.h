@@ -268,6 +268,30 @@ int helm3d_kaczswp(int variant, const double *wnr, const double *wni, const doub
 int helm3d_solve_cgmn(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
                       const helm3d_solve_opts *opts, helm3d_solve_info *info);
 
+/* ---- multigrid-preconditioned FGMRES ------------------------------------------ */
+
+/* The ML_GMRES preconditioner (ML_GMRES.m:34-72, opMultigrid.m:71-110,
+ * construct_multigrid.m, GMG_V.m) over device-resident levels.  ops[l] and
+ * smoothers[l] are two contexts holding the same level-l operator (level 0:
+ * the operator to solve; level l: the model coarsened to 2^l spacing with
+ * ceil(npml / 2^l) PML points, built by the caller); the transfers are the
+ * trilinear interpolation between consecutive PML-extended grids over
+ * linspace(0,1,n) per axis (fine2coarse.m:40-57, LinInterp1D.m) and 1/8 of its
+ * transpose.  All contexts are switched to the first one's stream. */
+typedef struct helm3d_mg helm3d_mg;
+int helm3d_mg_create(int nlevels, helm3d_ctx *const *ops, helm3d_ctx *const *smoothers,
+                     helm3d_mg **out);
+int helm3d_mg_destroy(helm3d_mg *mg);
+/* direction 0: out(level) += P in(level+1); 1: out(level+1) = P^T in(level) / 8 */
+int helm3d_mg_transfer(helm3d_mg *mg, int level, int direction, const void *in_dev, void *out_dev);
+/* FGMRES (FGMRES.m, flexible, opts->maxinnerit restart) on level 0
+ * preconditioned by one recursive V-cycle per step: smoother FGMRES(k[1]) x
+ * k[0] cycles (tol 1e-6), coarse solves FGMRES(k[3]) x k[2] cycles (tol 0.5)
+ * preconditioned by the next level's V-cycle, the coarsest by another such
+ * solve (k = NULL: 3, 5, 3, 5 as ML_GMRES.m:3-6). */
+int helm3d_solve_mg(helm3d_mg *mg, int mode, const void *b_dev, void *x_dev,
+                    const helm3d_solve_opts *opts, const int *k, helm3d_solve_info *info);
+
 /* ---- FWI gradient ------------------------------------------------------- */
 
 #define HELM3D_GRID_PHYS 0  /* g on the physical grid: comp2phys = truncation  */

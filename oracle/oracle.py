@@ -335,3 +335,37 @@ def cgmn(sweep, b, tol=1e-6, maxit=10000):
             break
         p = r + (normr / normr_old) ** 2 * p
     return x, k, np.array(hst)
+
+
+# ---------------------------------------------------------------------------
+# multigrid grid transfers
+# ---------------------------------------------------------------------------
+
+def lin_interp_1d(xin, xout):
+    """LinInterp1D.m: weights (len(xout) x len(xin)) of linear interpolation,
+    the bracket found from the nearest input point (stepped left when that
+    point lies right of the output point)."""
+    import scipy.sparse as sp
+    xin = np.sort(np.asarray(xin, dtype=np.float64).reshape(-1))
+    xout = np.sort(np.asarray(xout, dtype=np.float64).reshape(-1))
+    nin = xin.size
+    rows, cols, vals = [], [], []
+    for r, x in enumerate(xout):
+        I = int(np.argmin(np.abs(xin - x)))
+        if xin[I] > x:
+            I -= 1
+        x1 = xin[I + 1] if I + 1 < nin else 1e12
+        a = (x1 - x) / (x1 - xin[I])
+        b = (x - xin[I]) / (x1 - xin[I])
+        rows += [r, r]
+        cols += [I, min(I + 1, nin - 1)]
+        vals += [a, b]
+    return sp.csr_matrix((vals, (rows, cols)), shape=(xout.size, nin))
+
+
+def prolongation(nf, nc):
+    """c2f of fine2coarse(nf, nc, 'linear') (fine2coarse.m:49-53): the
+    trilinear interpolation from the coarse to the fine grid, x fastest."""
+    import scipy.sparse as sp
+    L = [lin_interp_1d(np.linspace(0, 1, nc[a]), np.linspace(0, 1, nf[a])) for a in range(3)]
+    return sp.kron(L[2], sp.kron(L[1], L[0])).tocsr()

@@ -198,6 +198,28 @@ inline int bind(helm3d_ctx* c)
 // every slot starts on a 4 KiB boundary.
 inline long slot_ld(long N) { return (N + 255) / 256 * 256; }
 
+// Flexible preconditioner hook of the GMRES engine: z = M^{-1} (s * v_raw)
+struct Precond {
+    virtual ~Precond() = default;
+    virtual int apply(const cpx<double>* v_raw, double s, cpx<double>* z) = 0;
+};
+
+struct GmresResult {
+    long iters = 0;
+    long cycles = 0;
+    double relres = 1.0;
+    double quasi = 1.0;
+    bool converged = false;
+};
+
+// Restarted GMRES of FGMRES.m:33-128 on the context's operator (variant) and
+// workspace (helm3d_capi.cpp).  P (complex128 only) makes it flexible.
+template <typename R>
+int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
+                 long maxit, long iter_cap, Precond* P, GmresResult* res, std::vector<double>* hst);
+
+int model_variant(const helm3d_ctx* c, int mode_adj);
+
 // Context work buffers: reduction partials, reduced values (+ pinned host
 // mirror) and the stencil's write-only sink.
 inline int ensure_work(helm3d_ctx* c)

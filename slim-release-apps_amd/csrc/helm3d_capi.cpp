@@ -98,11 +98,6 @@ int stencil(helm3d_ctx* c, int variant, const OpDesc<R>& op, const cpx<R>* x, cp
     return HELM3D_OK;
 }
 
-int model_variant(const helm3d_ctx* c, int mode_adj)
-{
-    return (mode_adj ? VAR_ADJ : VAR_FORW) | (c->wnc ? VAR_WNC : 0);
-}
-
 // Reduce `nq` partial columns (part[q*nblk + b]) into c->small[off..off+nq).
 int reduce_to_small(helm3d_ctx* c, int nblk, int nq, int off)
 {
@@ -117,19 +112,9 @@ int fetch_small(helm3d_ctx* c, int count)
     return HELM3D_OK;
 }
 
-// Flexible preconditioner hook: z = M^{-1} (s * v_raw)
-struct Precond {
-    virtual ~Precond() = default;
-    virtual int apply(const cpx<double>* v_raw, double s, cpx<double>* z) = 0;
-};
+}  // namespace
 
-struct GmresResult {
-    long iters = 0;
-    long cycles = 0;
-    double relres = 1.0;
-    double quasi = 1.0;
-    bool converged = false;
-};
+namespace helm3d {
 
 // Restarted GMRES of FGMRES.m:33-128 on precision R.  The basis is kept
 // unnormalised (slot c raw, scale sig[c]); P (R = double only) makes it the
@@ -252,6 +237,20 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     res->converged = res->relres < tol;
     return HELM3D_OK;
 }
+
+template int gmres_engine<double>(helm3d_ctx*, int, const cpx<double>*, cpx<double>*, bool, double, int, long, long,
+                                  Precond*, GmresResult*, std::vector<double>*);
+template int gmres_engine<float>(helm3d_ctx*, int, const cpx<float>*, cpx<float>*, bool, double, int, long, long,
+                                 Precond*, GmresResult*, std::vector<double>*);
+
+int model_variant(const helm3d_ctx* c, int mode_adj)
+{
+    return (mode_adj ? VAR_ADJ : VAR_FORW) | (c->wnc ? VAR_WNC : 0);
+}
+
+}  // namespace helm3d
+
+namespace {
 
 template <typename R>
 Vecs<R>& bicg_of(helm3d_ctx* c);

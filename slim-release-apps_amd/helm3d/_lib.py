@@ -123,6 +123,12 @@ _SIGS = [
                                       _dp, _dp, ctypes.c_longlong, _dp, _dp, ctypes.c_longlong, _dp, _dp]),
     ("helm3d_solve_cgmn", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
                                          ctypes.POINTER(SolveInfo)]),
+    ("helm3d_mg_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                        ctypes.POINTER(_vp)]),
+    ("helm3d_mg_destroy", ctypes.c_int, [_vp]),
+    ("helm3d_mg_transfer", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("helm3d_solve_mg", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts), _ip,
+                                       ctypes.POINTER(SolveInfo)]),
     ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
     ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),

@@ -25,7 +25,7 @@ class BatchSolver:
     on one device; reusable across calls."""
 
     def __init__(self, device=None):
-        device = torch.device(device if device is not None else ("cuda", torch.cuda.current_device()))
+        device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.device = device
         self._b = ctypes.c_void_p()
         with torch.cuda.device(device):

@@ -251,7 +251,7 @@ class FWIProblem:
     def __init__(self, model: FWIModel, opts: Optional[FWIOpts] = None, Q=None, device=None):
         self.model = model
         self.opts = opts or FWIOpts()
-        self.device = torch.device(device or ("cuda", torch.cuda.current_device()))
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.n = tuple(int(a) for a in model.n)
         self.d = tuple(float(a) for a in model.d)
         self.Q = np.eye(model.nsrc) if Q is None else np.asarray(Q)
