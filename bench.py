@@ -154,6 +154,52 @@ def mixed_leg(plain_rhs_per_s):
             "dtype": "c64 inner / f64 outer"}
 
 
+def uneven_leg(plain_rhs_per_s):
+    """The uneven batched workload: 8 subproblems (grids between 96^3 and
+    256x256x128 + 20-pt PML, 3-8 Hz, 4-32 impulses each), every right-hand
+    side solved to 1e-6 FP64 residual with batched mixed precision, split into
+    batches that fit the device."""
+    import torch
+    from helm3d import BatchSolver, Helm3d
+    from helm3d.models import config_c4, extend, wavenumber, vec
+    subs = config_c4()
+    dev = torch.device("cuda:0")
+    ops, bs, owner = [], [], []
+    for k, c in enumerate(subs):
+        wn = vec(wavenumber(extend(c.v, c.npml), c.freq))
+        H = Helm3d(c.nt, (c.h,) * 3, c.npml6, torch.from_numpy(wn).to(dev))
+        p, (nx, ny, _) = c.npml, c.nt
+        for s in c.sources:
+            b = torch.zeros(H.N, dtype=torch.complex128, device=dev)
+            b[(s[0] + p) + (s[1] + p) * nx + (s[2] + p) * nx * ny] = 1.0
+            ops.append(H)
+            bs.append(b)
+            owner.append(k)
+    S = BatchSolver(dev)
+    first = [owner.index(k) for k in range(len(subs))]  # one right-hand side of every subproblem
+    tuned = S.autotune([ops[i] for i in first], [bs[i] for i in first], inner_maxit=20, tile_rows=(4, 8),
+                       prefetch=(2,), divisors=(1, 2))
+    kw = dict(method="mixed", tol=1e-6, maxinnerit=20, inner_maxit=20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs, infos, chunks = S.solve_chunked(ops, bs, **kw)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rate = len(bs) / dt
+    per_sub = []
+    for k, c in enumerate(subs):
+        its = [infos[i].iters for i in range(len(bs)) if owner[i] == k]
+        per_sub.append({"n_phys": list(c.n_phys), "nt": list(c.nt), "freq": round(c.freq, 3), "rhs": len(its),
+                        "outer_iters": [min(its), max(its)]})
+    return {"workload": "8 subproblems, grids in [96^3, 256x256x128] + 20-pt PML, h=10 m, 3-8 Hz, 4-32 impulses each "
+                        f"({len(bs)} right-hand sides), batched mixed-precision solves to 1e-6 FP64 residual",
+            "rhs_per_s": rate, "seconds": dt, "vs_plain_c2": rate / plain_rhs_per_s,
+            "grid_point_rhs_per_s": sum(H.N for H in ops) / dt,
+            "batches": [len(ch) for ch in chunks], "converged": all(i.converged for i in infos),
+            "max_relres": max(i.relres for i in infos), "subproblems": per_sub, "launch": tuned,
+            "dtype": "c64 inner / f64 outer"}
+
+
 def fwi_leg():
     """One least-squares FWI gradient on the 256x256x128 workload."""
     import torch
@@ -233,6 +279,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-precision leg")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI gradient leg")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
     ap.add_argument("--mode", choices=("solve", "matvec", "copy"), default="solve",
@@ -349,6 +396,11 @@ def main():
     if not args.no_mixed:
         out["mixed_batched"] = mixed_leg(args.steps / elapsed)
         print(f"[bench] mixed leg done: {out['mixed_batched']['rhs_per_s']:.3f} RHS/s", file=sys.stderr, flush=True)
+    if not args.no_uneven:
+        out["batched_uneven"] = uneven_leg(args.steps / elapsed)
+        print(f"[bench] uneven batched leg done: {out['batched_uneven']['rhs_per_s']:.3f} RHS/s", file=sys.stderr,
+              flush=True)
+        torch.cuda.empty_cache()
     if not args.no_fwi:
         out["fwi_grads_per_min"], out["fwi"] = fwi_leg()
         print(f"[bench] fwi leg done: {out['fwi_grads_per_min']:.3f} grads/min", file=sys.stderr, flush=True)

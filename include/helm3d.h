@@ -25,6 +25,8 @@
 #ifndef HELM3D_H
 #define HELM3D_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -223,6 +225,7 @@ int helm3d_solve_bicgstab(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_
 typedef struct helm3d_batch helm3d_batch;
 
 #define HELM3D_BATCH_PLAIN 0  /* complex128 GMRES(maxinnerit), FGMRES_MT semantics          */
+#define HELM3D_BATCH_MAX_ITEMS 1024  /* items per helm3d_batch_solve call */
 #define HELM3D_BATCH_MIXED 1  /* FP64 flexible GMRES + inner_maxit complex64 BiCGStab        */
                               /* iterations per outer step; items stop individually        */
 
@@ -238,6 +241,11 @@ int helm3d_batch_set_stream(helm3d_batch *batch, void *stream);
 int helm3d_batch_solve(helm3d_batch *batch, int n, helm3d_ctx *const *ctx, int mode,
                        const void *const *b, void *const *x, int method,
                        const helm3d_solve_opts *opts, helm3d_solve_info *info);
+/* Device workspace one item on ctx adds to a batch solve (Krylov bases,
+ * inner vectors, partial sums), for callers that split a large set of
+ * right-hand sides into batches that fit the device.  No counterpart in the
+ * reference. */
+int helm3d_batch_item_bytes(const helm3d_ctx *ctx, int method, int maxinnerit, size_t *bytes);
 
 /* ---- Kaczmarz sweeps and CGMN ----------------------------------------------- */
 

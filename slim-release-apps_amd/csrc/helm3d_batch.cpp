@@ -176,6 +176,24 @@ int helm3d_batch_set_stream(helm3d_batch* b, void* stream)
     return HELM3D_OK;
 }
 
+int helm3d_batch_item_bytes(const helm3d_ctx* c, int method, int maxinnerit, size_t* bytes)
+{
+    if (!bytes) return fail(HELM3D_ERR_ARG, "null bytes");
+    *bytes = 0;
+    if (!c || !c->has_model) return fail(HELM3D_ERR_NOMODEL, "context without a model");
+    if (method != HELM3D_BATCH_PLAIN && method != HELM3D_BATCH_MIXED) return fail(HELM3D_ERR_ARG, "bad method");
+    if (maxinnerit < 1 || maxinnerit > kMaxBasis - 1) return fail(HELM3D_ERR_ARG, "maxinnerit must be in [1, 31]");
+    const bool mixed = method == HELM3D_BATCH_MIXED;
+    const size_t ld = size_t(slot_ld(c->N));
+    const size_t slots64 = size_t(maxinnerit + 1) + (mixed ? maxinnerit : 0);
+    const size_t slots32 = mixed ? S32_COUNT : 0;
+    const auto g = stencil_grid<double>(c->n[0], c->n[1], c->n[2], c->grid.kchunk, c->grid.ty, c->grid.pf);
+    const size_t part = size_t(std::max(krylov_blocks(c->N), g.blocks)) * 2 * kMaxBasis * sizeof(double);
+    *bytes = ld * (slots64 * sizeof(cpx<double>) + slots32 * sizeof(cpx<float>)) + part +
+             sizeof(double) * (kSmallStride + kSc + 2 * kMaxBasis) + 4096;
+    return HELM3D_OK;
+}
+
 int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode, const void* const* bs,
                        void* const* xs, int method, const helm3d_solve_opts* opts, helm3d_solve_info* info)
 {

@@ -38,6 +38,7 @@ HELM3D_OPT_GRAPHS = 1
 
 HELM3D_BATCH_PLAIN = 0
 HELM3D_BATCH_MIXED = 1
+HELM3D_BATCH_MAX_ITEMS = 1024  # items per helm3d_batch_solve call
 
 HELM3D_GRID_PHYS = 0
 HELM3D_GRID_PML = 1
@@ -118,6 +119,7 @@ _SIGS = [
     ("helm3d_batch_solve", ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int,
                                           ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int,
                                           ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
+    ("helm3d_batch_item_bytes", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
     ("helm3d_kacz_sweep", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     ("helm3d_kaczswp", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_longlong, _dp, ctypes.c_double,
                                       _dp, _dp, ctypes.c_longlong, _dp, _dp, ctypes.c_longlong, _dp, _dp]),

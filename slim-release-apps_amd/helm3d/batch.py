@@ -11,7 +11,7 @@ stopping individually.
 from __future__ import annotations
 
 import ctypes
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import torch
 
@@ -75,6 +75,55 @@ class BatchSolver:
         for H in uniq:
             H.set_launch((H.n[2] + d - 1) // d, ty, pf)
         return dict(tile_rows=ty, prefetch=pf, divisor=d, ms=best[0])
+
+    @staticmethod
+    def item_bytes(H: Helm3d, method: str = "mixed", maxinnerit: int = 20) -> int:
+        """Device workspace one right-hand side on H adds to a batch."""
+        meth = {"plain": _lib.HELM3D_BATCH_PLAIN, "mixed": _lib.HELM3D_BATCH_MIXED}[method]
+        out = ctypes.c_size_t()
+        check(lib().helm3d_batch_item_bytes(H._ctx, meth, int(maxinnerit), ctypes.byref(out)))
+        return int(out.value)
+
+    def plan(self, ops: Sequence[Helm3d], method: str = "mixed", maxinnerit: int = 20,
+             budget_bytes: Optional[int] = None) -> List[List[int]]:
+        """Split the items, in the given order, into consecutive batches whose
+        workspace fits ``budget_bytes`` (default: 85 % of the device memory
+        free now).  Items of one operator should be adjacent: a mixed batch
+        lasts as long as its slowest item."""
+        if budget_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            budget_bytes = int(0.85 * free)
+        cost = {}
+        chunks: List[List[int]] = []
+        used = 0
+        for i, H in enumerate(ops):
+            if id(H) not in cost:
+                cost[id(H)] = self.item_bytes(H, method, maxinnerit)
+            c = cost[id(H)]
+            if c > budget_bytes:
+                raise MemoryError(f"one right-hand side needs {c} B of workspace, budget {budget_bytes} B")
+            if not chunks or used + c > budget_bytes or len(chunks[-1]) >= _lib.HELM3D_BATCH_MAX_ITEMS:
+                chunks.append([])
+                used = 0
+            chunks[-1].append(i)
+            used += c
+        return chunks
+
+    def solve_chunked(self, ops: Sequence[Helm3d], bs: Sequence[torch.Tensor], mode: int = 1,
+                      method: str = "mixed", budget_bytes: Optional[int] = None, **kw
+                      ) -> Tuple[List[torch.Tensor], List[SolveInfo], List[List[int]]]:
+        """solve() over any number of right-hand sides: the items are split
+        by plan() into batches that fit the device, solved one batch after
+        the other.  Returns (xs, infos, batches)."""
+        maxinnerit = kw.get("maxinnerit", 20)
+        chunks = self.plan(ops, method, maxinnerit, budget_bytes)
+        xs: List[Optional[torch.Tensor]] = [None] * len(ops)
+        infos: List[Optional[SolveInfo]] = [None] * len(ops)
+        for ch in chunks:
+            x, inf = self.solve([ops[i] for i in ch], [bs[i] for i in ch], mode=mode, method=method, **kw)
+            for j, i in enumerate(ch):
+                xs[i], infos[i] = x[j], inf[j]
+        return xs, infos, chunks
 
     def solve(self, ops: Sequence[Helm3d], bs: Sequence[torch.Tensor], mode: int = 1, method: str = "mixed",
               tol: float = 1e-6, maxinnerit: int = 20, maxit: int = 10000, inner_maxit: int = 25,

@@ -11,6 +11,9 @@ order; flattening for the operator uses MATLAB column-major order (x fastest),
       times (1 + 0.1 s), s a sigma-8 smoothed field scaled to [-1,1];
       impulse at (100,100,5).
   C3  C2 model, 8 impulses at z=5, y=100, x=round(linspace(20,179,8)).
+  C4  8 subproblems, grids drawn in [96^3, 256x256x128] + 20-pt PML, h 10 m,
+      frequencies uniform in 3-8 Hz, each with the C2 velocity recipe on its
+      own grid and 4-32 impulses on a seeded surface grid (z = 5).
   C5  256x256x128 + 20-pt PML, h 12.5 m: smooth background (C1 recipe with
       sigma 20) plus 12 Gaussian blobs (+-300 m/s, radius 10-25 pts); start
       model = true model smoothed with sigma 20.
@@ -82,6 +85,46 @@ def config_c2(seed: int = SEED_BASE + 2) -> Config:
 def config_c3(seed: int = SEED_BASE + 2) -> Config:
     xs = np.round(np.linspace(20, 179, 8)).astype(int)
     return Config("C3", (200, 200, 100), 20, 10.0, 6.0, _c2_velocity(seed), [(int(x), 100, 5) for x in xs])
+
+
+def _gradient_velocity(n, rng) -> np.ndarray:
+    """C2 recipe on any grid: 1500->4000 m/s linear in depth times
+    (1 + 0.1 s), s a sigma-8 smoothed field scaled to [-1,1]."""
+    z = np.arange(n[2], dtype=np.float64)
+    vz = 1500.0 + 2500.0 * z / max(n[2] - 1, 1)
+    s = _scaled_smooth(n, 8, rng, -1.0, 1.0)
+    return vz[None, None, :] * (1.0 + 0.1 * s)
+
+
+def config_c4(seed: int = SEED_BASE + 4, scale: float = 1.0, nsub: int = 8) -> List[Config]:
+    """Uneven batched workload: ``nsub`` subproblems with (nx, ny) drawn
+    uniformly in [96, 256] and nz in [96, 128] (so every grid lies between
+    96^3 and 256x256x128), 20-point PML, h = 10 m, frequency uniform in
+    [3, 8] Hz, and 4-32 unit impulses at depth index 5 on distinct nodes of
+    a seeded surface grid (spacing 8 points, offset 4).  ``scale`` < 1 shrinks
+    every grid, the PML and the source spacing for quick parity runs (the
+    number of sources is kept)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nsub):
+        n = [int(rng.integers(96, 257)), int(rng.integers(96, 257)), int(rng.integers(96, 129))]
+        freq = float(rng.uniform(3.0, 8.0))
+        nsrc = int(rng.integers(4, 33))
+        sub_seed = int(rng.integers(0, 2 ** 31))
+        if scale != 1.0:
+            n = [max(12, int(round(a * scale))) for a in n]
+        npml = max(4, int(round(20 * min(scale, 1.0))))
+        sp = max(2, int(round(8 * min(scale, 1.0))))
+        srng = np.random.default_rng(sub_seed)
+        v = _gradient_velocity(tuple(n), srng)
+        gx = np.arange(sp // 2, n[0] - sp // 2, sp)
+        gy = np.arange(sp // 2, n[1] - sp // 2, sp)
+        nodes = [(int(x), int(y)) for y in gy for x in gx]
+        pick = srng.choice(len(nodes), size=min(nsrc, len(nodes)), replace=False)
+        zs = min(5, n[2] - 1)
+        srcs = [(nodes[i][0], nodes[i][1], zs) for i in sorted(pick)]
+        out.append(Config(f"C4.{k}", tuple(n), npml, 10.0, freq, v, srcs))
+    return out
 
 
 def config_small(n=(32, 32, 24), npml=8, h=10.0, freq=5.0, seed: int = SEED_BASE + 9) -> Config:

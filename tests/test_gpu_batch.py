@@ -145,3 +145,40 @@ def test_batch_autotune_keeps_results():
     for a, c, ia, ic in zip(x0, x1, i0, i1):
         assert ic.converged and abs(ia.iters - ic.iters) <= 2
         assert float((a - c).norm() / a.norm()) < 1e-4
+
+
+def test_uneven_workload_chunked():
+    """The uneven batched workload (8 subproblems with different grids and
+    frequencies, 4-32 impulses each) at reduced size: every right-hand side
+    reaches 1e-6 with the oracle operator, and splitting the items into
+    several batches under a small workspace budget gives the same bits as one
+    batch (items of a mixed batch are independent)."""
+    import torch
+    from helm3d import BatchSolver, Helm3d
+    from helm3d.models import config_c4, extend, point_source, vec, wavenumber
+    subs = config_c4(scale=0.2)
+    ops, bs, meta = [], [], []
+    for c in subs:
+        wn = vec(wavenumber(extend(c.v, c.npml), c.freq))
+        H = Helm3d(c.nt, (c.h,) * 3, c.npml6, torch.from_numpy(wn).to("cuda:0"))
+        for s in c.sources:
+            ops.append(H)
+            bs.append(torch.from_numpy(point_source(c, s)).cuda())
+            meta.append((c, wn))
+    print("uneven workload:", len(subs), "grids,", len(bs), "right-hand sides", flush=True)
+    S = BatchSolver("cuda:0")
+    per = BatchSolver.item_bytes(ops[0], "mixed", 20)
+    assert per > 41 * 16 * ops[0].N
+    budget = sum(BatchSolver.item_bytes(H, "mixed", 20) for H in ops) // 3 + 1
+    xs, infos, chunks = S.solve_chunked(ops, bs, method="mixed", tol=1e-6, inner_maxit=10, budget_bytes=budget)
+    print("batches", [len(ch) for ch in chunks], flush=True)
+    assert len(chunks) >= 3 and sorted(i for ch in chunks for i in ch) == list(range(len(bs)))
+    worst = 0.0
+    for (c, wn), b, x, inf in zip(meta, bs, xs, infos):
+        rel = _relres(c, wn, b.cpu().numpy(), x.cpu().numpy(), 1)
+        worst = max(worst, rel)
+        assert inf.converged and rel < 1e-6 and abs(rel - inf.relres) < 1e-9
+    print("worst relres", worst, "outer iters", min(i.iters for i in infos), "-", max(i.iters for i in infos))
+    x1, i1 = S.solve(ops, bs, method="mixed", tol=1e-6, inner_maxit=10)
+    for a, b_, ia, ib in zip(xs, x1, infos, i1):
+        assert ia.iters == ib.iters and torch.equal(a, b_)
