@@ -148,7 +148,7 @@ def mixed_leg(plain_rhs_per_s):
     rate = len(bs) / dt
     return {"workload": "8 point sources (z=5, y=100, x=round(linspace(20,179,8))) on the 240x240x140 benchmark "
                         "model, one batched mixed-precision solve to 1e-6 FP64 residual",
-            "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s,
+            "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s if plain_rhs_per_s else None,
             "outer_iters": [i.iters for i in infos], "inner_iters_per_outer": 20,
             "final_relres": [i.relres for i in infos], "err_vs_complex128": errs, "launch": tuned,
             "dtype": "c64 inner / f64 outer"}
@@ -193,22 +193,70 @@ def uneven_leg(plain_rhs_per_s):
                         "outer_iters": [min(its), max(its)]})
     return {"workload": "8 subproblems, grids in [96^3, 256x256x128] + 20-pt PML, h=10 m, 3-8 Hz, 4-32 impulses each "
                         f"({len(bs)} right-hand sides), batched mixed-precision solves to 1e-6 FP64 residual",
-            "rhs_per_s": rate, "seconds": dt, "vs_plain_c2": rate / plain_rhs_per_s,
+            "rhs_per_s": rate, "seconds": dt,
+            "vs_plain_c2": rate / plain_rhs_per_s if plain_rhs_per_s else None,
             "grid_point_rhs_per_s": sum(H.N for H in ops) / dt,
             "batches": [len(ch) for ch in chunks], "converged": all(i.converged for i in infos),
             "max_relres": max(i.relres for i in infos), "subproblems": per_sub, "launch": tuned,
             "dtype": "c64 inner / f64 outer"}
 
 
-def fwi_leg():
-    """One least-squares FWI gradient on the 256x256x128 workload."""
+def mg_leg(plain_rhs_per_s, solves=3):
+    """The benchmark solve (240x240x140, 6 Hz, one impulse, 1e-6 FP64
+    residual) with FGMRES(20) preconditioned by the ML_GMRES V-cycle: 3 levels
+    (ML_GMRES.m) and 4 levels, the V-cycle in complex128 and in complex64."""
+    import torch
+    from helm3d.mg import MLGMRES
+    from helm3d.models import config_c2, vec
+    cfg = config_c2()
+    dev = torch.device("cuda:0")
+    runs = []
+    for levels in (3, 4):
+        M = MLGMRES(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=levels, device=dev)
+        nx, ny, _ = M.grids[0]["nt"]
+        p, s = cfg.npml, cfg.sources[0]
+        b = torch.zeros(M.H.N, dtype=torch.complex128, device=dev)
+        b[(s[0] + p) + (s[1] + p) * nx + (s[2] + p) * nx * ny] = 1.0
+        for prec in ("c128", "mixed"):
+            M.solve(b, tol=1e-6, precision=prec)  # warm-up (workspaces)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(solves):
+                x, info = M.solve(b, tol=1e-6, precision=prec)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / solves
+            relres = float((b - M.H.apply(x)).norm() / b.norm())
+            runs.append({"levels": levels, "grids": [list(g["nt"]) for g in M.grids],
+                         "vcycle": "complex128" if prec == "c128" else "complex64", "rhs_per_s": 1.0 / dt,
+                         "seconds_per_solve": dt,
+                         "vs_plain": (1.0 / dt) / plain_rhs_per_s if plain_rhs_per_s else None,
+                         "outer_iters": info.iters, "final_relres": relres})
+        M.close()
+        del M
+        torch.cuda.empty_cache()
+    best = max(runs, key=lambda r: r["rhs_per_s"])
+    return {"workload": "the 240x240x140 benchmark solve (one impulse, 6 Hz, 1e-6 FP64 residual) with FGMRES(20) "
+                        "preconditioned by the ML_GMRES V-cycle, k = (3, 5, 3, 5)",
+            "rhs_per_s": best["rhs_per_s"], "best": {k: best[k] for k in ("levels", "vcycle")}, "runs": runs,
+            "dtype": "f64 outer / c128 or c64 V-cycle"}
+
+
+def fwi_leg(solver="mg"):
+    """One least-squares FWI gradient on the 256x256x128 workload; every
+    solve to 1e-6 with FGMRES preconditioned by a 4-level complex64 V-cycle
+    (solver "mg"), or with batched mixed-precision solves ("mixed")."""
     import torch
     from helm3d.fwi import FWIOpts, FWIProblem
     from helm3d.models import fwi_c5
     model, m_true, m0 = fwi_c5()
     dev = torch.device("cuda:0")
-    P = FWIProblem(model, FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20),
-                   device=dev)
+    if solver == "mg":
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed")
+        how = "FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, FP64 outer"
+    else:
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20)
+        how = "batched mixed-precision solves (16 sources per batch)"
+    P = FWIProblem(model, opts, device=dev)
     D = P.forw_model(m_true)  # observed data (not timed)
     P.opts.stats.clear()
     torch.cuda.synchronize()
@@ -218,7 +266,7 @@ def fwi_leg():
     dt = time.perf_counter() - t0
     its = [i.iters for i in P.opts.stats]
     return 60.0 / dt, {"workload": "LS-FWI gradient w.r.t. s2/m2, 256x256x128 + 20-pt PML at 12.5 m, 16 sources, "
-                                   "64x64 receivers, 4 and 6 Hz, batched mixed solves to 1e-6",
+                                   f"64x64 receivers, 4 and 6 Hz, solves to 1e-6: {how}",
                        "seconds_per_gradient": dt, "solves": len(its), "outer_iters_mean": sum(its) / len(its),
                        "misfit": f, "grad_norm": float(g.norm())}
 
@@ -280,11 +328,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched mixed-precision leg")
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI gradient leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched leg")
+    ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
-    ap.add_argument("--mode", choices=("solve", "matvec", "copy"), default="solve",
+    ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg"), default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
-                         "(profiling workload); copy: K device copies of 256 MiB (PMC calibration)")
+                         "(profiling workload); copy: K device copies of 256 MiB (PMC calibration); "
+                         "mixed / uneven / fwi / mg: one of the side legs alone (profiling workloads)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -301,6 +351,18 @@ def main():
 
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode in ("mixed", "uneven", "fwi", "mg"):
+        plain_rhs_per_s = None  # no plain solve in this run: ratios are left out
+        if args.mode == "mixed":
+            res = mixed_leg(plain_rhs_per_s)
+        elif args.mode == "uneven":
+            res = uneven_leg(plain_rhs_per_s)
+        elif args.mode == "mg":
+            res = mg_leg(plain_rhs_per_s)
+        else:
+            res = dict(zip(("grads_per_min", "fwi"), fwi_leg()))
+        print(json.dumps({"mode": args.mode, **res}))
+        return
     cfg = config_c2()
     wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
     if args.mode == "matvec":
@@ -396,6 +458,10 @@ def main():
     if not args.no_mixed:
         out["mixed_batched"] = mixed_leg(args.steps / elapsed)
         print(f"[bench] mixed leg done: {out['mixed_batched']['rhs_per_s']:.3f} RHS/s", file=sys.stderr, flush=True)
+    if not args.no_mg:
+        out["multigrid"] = mg_leg(args.steps / elapsed)
+        print(f"[bench] multigrid leg done: {out['multigrid']['rhs_per_s']:.3f} RHS/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     if not args.no_uneven:
         out["batched_uneven"] = uneven_leg(args.steps / elapsed)
         print(f"[bench] uneven batched leg done: {out['batched_uneven']['rhs_per_s']:.3f} RHS/s", file=sys.stderr,

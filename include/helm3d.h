@@ -299,6 +299,12 @@ int helm3d_mg_transfer(helm3d_mg *mg, int level, int direction, const void *in_d
  * solve (k = NULL: 3, 5, 3, 5 as ML_GMRES.m:3-6). */
 int helm3d_solve_mg(helm3d_mg *mg, int mode, const void *b_dev, void *x_dev,
                     const helm3d_solve_opts *opts, const int *k, helm3d_solve_info *info);
+/* Mixed precision form of helm3d_solve_mg: the outer FGMRES, its basis, the
+ * Hessenberg least squares and the true residuals in FP64; the V-cycle
+ * preconditioner (smoothers, residuals, transfers, coarse solves) in
+ * complex64 with the FP32 copy of every level's operator.  b, x complex128. */
+int helm3d_solve_mg_mixed(helm3d_mg *mg, int mode, const void *b_dev, void *x_dev,
+                          const helm3d_solve_opts *opts, const int *k, helm3d_solve_info *info);
 
 /* ---- FWI gradient ------------------------------------------------------- */
 

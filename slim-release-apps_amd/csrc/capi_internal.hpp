@@ -152,7 +152,7 @@ struct helm3d_ctx {
     double* small_host = nullptr;  // pinned mirror
 
     Vecs<double> V64, Z64, T64, BC64;
-    Vecs<float> V32, T32a, T32b, BC32;
+    Vecs<float> V32, Z32, T32a, T32b, BC32;
     double* bsc = nullptr;  // BiCGStab device scalars (BS_SLOTS doubles)
     size_t bsc_cap = 0;
 
@@ -175,7 +175,7 @@ struct helm3d_ctx {
     {
         (void)hipSetDevice(device);
         void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, BC64.p,
-                        V32.p, T32a.p, T32b.p, BC32.p, bsc};
+                        V32.p, Z32.p, T32a.p, T32b.p, BC32.p, bsc};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (small_host) (void)hipHostFree(small_host);
@@ -198,11 +198,14 @@ inline int bind(helm3d_ctx* c)
 // every slot starts on a 4 KiB boundary.
 inline long slot_ld(long N) { return (N + 255) / 256 * 256; }
 
-// Flexible preconditioner hook of the GMRES engine: z = M^{-1} (s * v_raw)
-struct Precond {
-    virtual ~Precond() = default;
-    virtual int apply(const cpx<double>* v_raw, double s, cpx<double>* z) = 0;
+// Flexible preconditioner hook of the GMRES engine: z = M^{-1} (s * v_raw),
+// in the engine's precision.
+template <typename R>
+struct PrecondT {
+    virtual ~PrecondT() = default;
+    virtual int apply(const cpx<R>* v_raw, double s, cpx<R>* z) = 0;
 };
+using Precond = PrecondT<double>;
 
 struct GmresResult {
     long iters = 0;
@@ -213,10 +216,10 @@ struct GmresResult {
 };
 
 // Restarted GMRES of FGMRES.m:33-128 on the context's operator (variant) and
-// workspace (helm3d_capi.cpp).  P (complex128 only) makes it flexible.
+// workspace (helm3d_capi.cpp).  P makes it flexible.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
-                 long maxit, long iter_cap, Precond* P, GmresResult* res, std::vector<double>* hst);
+                 long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst);
 
 int model_variant(const helm3d_ctx* c, int mode_adj);
 

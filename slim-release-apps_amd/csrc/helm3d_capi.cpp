@@ -78,6 +78,13 @@ Vecs<double>& basis_of<double>(helm3d_ctx* c) { return c->V64; }
 template <>
 Vecs<float>& basis_of<float>(helm3d_ctx* c) { return c->V32; }
 
+template <typename R>
+Vecs<R>& zbasis_of(helm3d_ctx* c);
+template <>
+Vecs<double>& zbasis_of<double>(helm3d_ctx* c) { return c->Z64; }
+template <>
+Vecs<float>& zbasis_of<float>(helm3d_ctx* c) { return c->Z32; }
+
 
 
 // Stencil launch on the context's operator; y = alpha*Hx + beta*b.
@@ -117,11 +124,11 @@ int fetch_small(helm3d_ctx* c, int count)
 namespace helm3d {
 
 // Restarted GMRES of FGMRES.m:33-128 on precision R.  The basis is kept
-// unnormalised (slot c raw, scale sig[c]); P (R = double only) makes it the
-// flexible variant of FGMRES.m:96-98,118 with Z stored true-scaled.
+// unnormalised (slot c raw, scale sig[c]); P makes it the flexible variant of
+// FGMRES.m:96-98,118 with Z stored true-scaled.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
-                 long maxit, long iter_cap, Precond* P, GmresResult* res, std::vector<double>* hst)
+                 long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst)
 {
     using C = std::complex<double>;
     if (m < 1 || m > kMaxBasis - 1) return fail(HELM3D_ERR_ARG, "maxinnerit must be in [1, 31]");
@@ -132,10 +139,11 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     CHK(ensure_vecs(Vb, N, m + 1));
     CHK(ensure_work(c));
     cpx<R>* V = Vb.p;
-    cpx<double>* Z = nullptr;
+    cpx<R>* Z = nullptr;
     if (P) {
-        CHK(ensure_vecs(c->Z64, N, m));
-        Z = c->Z64.p;
+        Vecs<R>& Zb = zbasis_of<R>(c);
+        CHK(ensure_vecs(Zb, N, m));
+        Z = Zb.p;
     }
     const int nblk = krylov_blocks(N);
     auto slot = [&](int s) { return V + size_t(s) * ld; };
@@ -174,10 +182,8 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         while (k <= m && hlast > tol) {
             // w = A v_k (or A z_k) into slot k
             if (P) {
-                if constexpr (std::is_same<R, double>::value) {
-                    CHK(P->apply(slot(k - 1), sig[k - 1], Z + size_t(k - 1) * ld));
-                    CHK(stencil<R>(c, variant, op, Z + size_t(k - 1) * ld, slot(k), R(1), nullptr, R(0), nullptr));
-                }
+                CHK(P->apply(slot(k - 1), sig[k - 1], Z + size_t(k - 1) * ld));
+                CHK(stencil<R>(c, variant, op, Z + size_t(k - 1) * ld, slot(k), R(1), nullptr, R(0), nullptr));
             } else {
                 CHK(stencil<R>(c, variant, op, slot(k - 1), slot(k), R(sig[k - 1]), nullptr, R(0), nullptr));
             }
@@ -216,8 +222,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             a.v[2 * q + 1] = v.imag();
         }
         if (P) {
-            if constexpr (std::is_same<R, double>::value)
-                TIMED(HELM3D_PROF_COMBINE, launch_combine<double>(Z, ld, k - 1, a, x, N, nblk, c->stream));
+            TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(Z, ld, k - 1, a, x, N, nblk, c->stream));
         } else {
             TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(V, ld, k - 1, a, x, N, nblk, c->stream));
         }
@@ -239,9 +244,9 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
 }
 
 template int gmres_engine<double>(helm3d_ctx*, int, const cpx<double>*, cpx<double>*, bool, double, int, long, long,
-                                  Precond*, GmresResult*, std::vector<double>*);
+                                  PrecondT<double>*, GmresResult*, std::vector<double>*);
 template int gmres_engine<float>(helm3d_ctx*, int, const cpx<float>*, cpx<float>*, bool, double, int, long, long,
-                                 Precond*, GmresResult*, std::vector<double>*);
+                                 PrecondT<float>*, GmresResult*, std::vector<double>*);
 
 int model_variant(const helm3d_ctx* c, int mode_adj)
 {

@@ -14,7 +14,10 @@
 //                                               preconditioned by V_{l+1} (l+1 < L-1)
 //                                               or by FGMRES(kc) on A_{L-1} (l+1 = L-1)
 //            x = x + P_l y ;  x = S_l(b, x)     (GMG_V.m:23-31, one cycle)
-// and the outer solve is FGMRES on A_0 preconditioned by V_0.
+// and the outer solve is FGMRES on A_0 preconditioned by V_0.  The mixed form
+// (helm3d_solve_mg_mixed) runs the whole V-cycle -- smoothers, residuals,
+// transfers, coarse solves -- in complex64 with each context's FP32 operator,
+// under the FP64 outer FGMRES.
 #include "capi_internal.hpp"
 #include "mg.hpp"
 
@@ -102,85 +105,160 @@ struct MgParams {
     double smooth_tol = 1e-6, coarse_tol = 0.5;
 };
 
-struct VCycle;
+// Level buffers viewed in the V-cycle's precision (a complex64 vector fits in
+// a complex128 slot).
+template <typename R>
+cpx<R>* as(cpx<double>* p)
+{
+    return reinterpret_cast<cpx<R>*>(p);
+}
 
-int vcycle(helm3d_mg* mg, const MgParams& k, int l, int mode, const cpx<double>* b, cpx<double>* x);
+template <typename R>
+int vcycle(helm3d_mg* mg, const MgParams& k, int l, int mode, const cpx<R>* b, cpx<R>* x);
 
 // Precondition with V_l: z = V_l(s * v)
-struct VCycle : Precond {
+template <typename R>
+struct VCycle : PrecondT<R> {
     helm3d_mg* mg;
     const MgParams* k;
     int l, mode;
-    int apply(const cpx<double>* v, double s, cpx<double>* z) override
+    int apply(const cpx<R>* v, double s, cpx<R>* z) override
     {
         MgLevel& L = mg->lev[l];
-        HIPCHK((launch_axpby<double, double>(s, 0.0, v, 0.0, 0.0, L.vb, L.N, false, mg->stream)));
-        return vcycle(mg, *k, l, mode, L.vb, z);
+        HIPCHK((launch_axpby<R, R>(s, 0.0, v, 0.0, 0.0, as<R>(L.vb), L.N, false, mg->stream)));
+        return vcycle<R>(mg, *k, l, mode, as<R>(L.vb), z);
     }
 };
 
 // Precondition the coarsest solve with another FGMRES on the coarsest level
 // (construct: ls_opts.precond = LinearSolver(H{end}, coarse options)).
-struct CoarseInner : Precond {
+template <typename R>
+struct CoarseInner : PrecondT<R> {
+    helm3d_mg* mg;
+    const MgParams* k;
+    int mode;
+    int apply(const cpx<R>* v, double s, cpx<R>* z) override
+    {
+        MgLevel& L = mg->lev[mg->L - 1];
+        HIPCHK((launch_axpby<R, R>(s, 0.0, v, 0.0, 0.0, as<R>(L.vb), L.N, false, mg->stream)));
+        GmresResult r;
+        return gmres_engine<R>(L.S, model_variant(L.S, mode != 1), as<R>(L.vb), z, false, k->coarse_tol,
+                               k->kc_inner, k->kc_outer, 0, nullptr, &r, nullptr);
+    }
+};
+
+// Mixed precision: the FP64 outer FGMRES hands s * v to a complex64 V-cycle
+// and takes its result back in complex128.
+struct MixedTop : Precond {
     helm3d_mg* mg;
     const MgParams* k;
     int mode;
     int apply(const cpx<double>* v, double s, cpx<double>* z) override
     {
-        MgLevel& L = mg->lev[mg->L - 1];
-        HIPCHK((launch_axpby<double, double>(s, 0.0, v, 0.0, 0.0, L.vb, L.N, false, mg->stream)));
-        GmresResult r;
-        return gmres_engine<double>(L.S, model_variant(L.S, mode != 1), L.vb, z, false, k->coarse_tol, k->kc_inner,
-                                    k->kc_outer, 0, nullptr, &r, nullptr);
+        MgLevel& L = mg->lev[0];
+        cpx<float>* b32 = as<float>(L.vb);
+        cpx<float>* x32 = as<float>(L.cx);
+        HIPCHK((launch_axpby<double, float>(s, 0.0, v, 0.0, 0.0, b32, L.N, false, mg->stream)));
+        CHK(vcycle<float>(mg, *k, 0, mode, b32, x32));
+        HIPCHK((launch_axpby<float, double>(1.0, 0.0, x32, 0.0, 0.0, z, L.N, false, mg->stream)));
+        return HELM3D_OK;
     }
 };
 
-int residual(MgLevel& L, int mode, const cpx<double>* b, const cpx<double>* x, cpx<double>* r)
+template <typename R>
+int residual(MgLevel& L, int mode, const cpx<R>* b, const cpx<R>* x, cpx<R>* r)
 {
-    StencilLaunch<double> S{};
-    S.op = L.A->op64;
+    StencilLaunch<R> S{};
+    if constexpr (std::is_same<R, double>::value)
+        S.op = L.A->op64;
+    else
+        S.op = L.A->op32;
     S.x = x;
     S.y = r;
     S.b = b;
-    S.alpha = -1.0;
-    S.beta = 1.0;
-    S.sink = static_cast<cpx<double>*>(L.A->sink);
-    HIPCHK(launch_stencil<double>(model_variant(L.A, mode != 1), S, L.A->grid, L.A->stream));
+    S.alpha = R(-1);
+    S.beta = R(1);
+    S.sink = static_cast<cpx<R>*>(L.A->sink);
+    HIPCHK(launch_stencil<R>(model_variant(L.A, mode != 1), S, L.A->grid, L.A->stream));
     return HELM3D_OK;
 }
 
-int vcycle(helm3d_mg* mg, const MgParams& k, int l, int mode, const cpx<double>* b, cpx<double>* x)
+template <typename R>
+int vcycle(helm3d_mg* mg, const MgParams& k, int l, int mode, const cpx<R>* b, cpx<R>* x)
 {
     MgLevel& L = mg->lev[l];
     MgLevel& C = mg->lev[l + 1];
     GmresResult r;
     // pre-smoothing from zero
-    CHK(gmres_engine<double>(L.S, model_variant(L.S, mode != 1), b, x, false, k.smooth_tol, k.ks_inner, k.ks_outer, 0,
-                             nullptr, &r, nullptr));
-    CHK(residual(L, mode, b, x, L.vr));
-    HIPCHK(launch_restrict(L.T, 0.125, L.vr, C.crhs, mg->stream));
+    CHK(gmres_engine<R>(L.S, model_variant(L.S, mode != 1), b, x, false, k.smooth_tol, k.ks_inner, k.ks_outer, 0,
+                        nullptr, &r, nullptr));
+    CHK(residual<R>(L, mode, b, x, as<R>(L.vr)));
+    HIPCHK(launch_restrict<R>(L.T, 0.125, as<R>(L.vr), as<R>(C.crhs), mg->stream));
     // coarse-grid solve, preconditioned by the next level
     if (l + 1 < mg->L - 1) {
-        VCycle P;
+        VCycle<R> P;
         P.mg = mg;
         P.k = &k;
         P.l = l + 1;
         P.mode = mode;
-        CHK(gmres_engine<double>(C.A, model_variant(C.A, mode != 1), C.crhs, C.cx, false, k.coarse_tol, k.kc_inner,
-                                 k.kc_outer, 0, &P, &r, nullptr));
+        CHK(gmres_engine<R>(C.A, model_variant(C.A, mode != 1), as<R>(C.crhs), as<R>(C.cx), false, k.coarse_tol,
+                            k.kc_inner, k.kc_outer, 0, &P, &r, nullptr));
     } else {
-        CoarseInner P;
+        CoarseInner<R> P;
         P.mg = mg;
         P.k = &k;
         P.mode = mode;
-        CHK(gmres_engine<double>(C.A, model_variant(C.A, mode != 1), C.crhs, C.cx, false, k.coarse_tol, k.kc_inner,
-                                 k.kc_outer, 0, &P, &r, nullptr));
+        CHK(gmres_engine<R>(C.A, model_variant(C.A, mode != 1), as<R>(C.crhs), as<R>(C.cx), false, k.coarse_tol,
+                            k.kc_inner, k.kc_outer, 0, &P, &r, nullptr));
     }
-    HIPCHK(launch_prolong_add(L.T, C.cx, x, mg->stream));
+    HIPCHK(launch_prolong_add<R>(L.T, as<R>(C.cx), x, mg->stream));
     // post-smoothing from the corrected x
-    CHK(gmres_engine<double>(L.S, model_variant(L.S, mode != 1), b, x, true, k.smooth_tol, k.ks_inner, k.ks_outer, 0,
-                             nullptr, &r, nullptr));
+    CHK(gmres_engine<R>(L.S, model_variant(L.S, mode != 1), b, x, true, k.smooth_tol, k.ks_inner, k.ks_outer, 0,
+                        nullptr, &r, nullptr));
     return HELM3D_OK;
+}
+
+int solve_mg(helm3d_mg* mg, bool mixed, int mode, const void* b, void* x, const helm3d_solve_opts* opts, const int* k,
+             helm3d_solve_info* info)
+{
+    if (!mg || !b || !x) return fail(HELM3D_ERR_ARG, "null argument");
+    HIPCHK(hipSetDevice(mg->device));
+    helm3d_solve_opts o;
+    helm3d_solve_opts_default(&o);
+    if (opts) o = *opts;
+    MgParams prm;
+    if (k) {
+        if (k[0] < 1 || k[1] < 1 || k[2] < 1 || k[3] < 1 || k[1] >= kMaxBasis || k[3] >= kMaxBasis)
+            return fail(HELM3D_ERR_ARG, "bad multigrid cycle sizes");
+        prm.ks_outer = k[0];
+        prm.ks_inner = k[1];
+        prm.kc_outer = k[2];
+        prm.kc_inner = k[3];
+    }
+    for (auto& L : mg->lev) {
+        L.A->stream = mg->stream;
+        L.S->stream = mg->stream;
+    }
+    MgLevel& F = mg->lev[0];
+    VCycle<double> P64;
+    MixedTop P32;
+    P64.mg = P32.mg = mg;
+    P64.k = P32.k = &prm;
+    P64.l = 0;
+    P64.mode = P32.mode = mode;
+    GmresResult r;
+    CHK(gmres_engine<double>(F.A, model_variant(F.A, mode != 1), static_cast<const cpx<double>*>(b),
+                             static_cast<cpx<double>*>(x), o.use_x0 != 0, o.tol, o.maxinnerit, long(o.maxit), 0,
+                             mixed ? static_cast<Precond*>(&P32) : static_cast<Precond*>(&P64), &r, nullptr));
+    if (info) {
+        info->iters = r.iters;
+        info->inner_iters = 0;
+        info->cycles = r.cycles;
+        info->relres = r.relres;
+        info->quasi_relres = r.quasi;
+        info->converged = r.converged;
+    }
+    return r.converged ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
 }
 
 }  // namespace
@@ -299,10 +377,11 @@ int helm3d_mg_transfer(helm3d_mg* mg, int level, int direction, const void* in, 
     HIPCHK(hipSetDevice(mg->device));
     const MgAxes& T = mg->lev[level].T;
     if (direction == 0) {
-        HIPCHK(launch_prolong_add(T, static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out), mg->stream));
+        HIPCHK(launch_prolong_add<double>(T, static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out),
+                                          mg->stream));
     } else {
-        HIPCHK(launch_restrict(T, 0.125, static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out),
-                               mg->stream));
+        HIPCHK(launch_restrict<double>(T, 0.125, static_cast<const cpx<double>*>(in), static_cast<cpx<double>*>(out),
+                                       mg->stream));
     }
     return HELM3D_OK;
 }
@@ -310,43 +389,13 @@ int helm3d_mg_transfer(helm3d_mg* mg, int level, int direction, const void* in, 
 int helm3d_solve_mg(helm3d_mg* mg, int mode, const void* b, void* x, const helm3d_solve_opts* opts, const int* k,
                     helm3d_solve_info* info)
 {
-    if (!mg || !b || !x) return fail(HELM3D_ERR_ARG, "null argument");
-    HIPCHK(hipSetDevice(mg->device));
-    helm3d_solve_opts o;
-    helm3d_solve_opts_default(&o);
-    if (opts) o = *opts;
-    MgParams prm;
-    if (k) {
-        if (k[0] < 1 || k[1] < 1 || k[2] < 1 || k[3] < 1 || k[1] >= kMaxBasis || k[3] >= kMaxBasis)
-            return fail(HELM3D_ERR_ARG, "bad multigrid cycle sizes");
-        prm.ks_outer = k[0];
-        prm.ks_inner = k[1];
-        prm.kc_outer = k[2];
-        prm.kc_inner = k[3];
-    }
-    for (auto& L : mg->lev) {
-        L.A->stream = mg->stream;
-        L.S->stream = mg->stream;
-    }
-    MgLevel& F = mg->lev[0];
-    VCycle P;
-    P.mg = mg;
-    P.k = &prm;
-    P.l = 0;
-    P.mode = mode;
-    GmresResult r;
-    CHK(gmres_engine<double>(F.A, model_variant(F.A, mode != 1), static_cast<const cpx<double>*>(b),
-                             static_cast<cpx<double>*>(x), o.use_x0 != 0, o.tol, o.maxinnerit, long(o.maxit), 0, &P,
-                             &r, nullptr));
-    if (info) {
-        info->iters = r.iters;
-        info->inner_iters = 0;
-        info->cycles = r.cycles;
-        info->relres = r.relres;
-        info->quasi_relres = r.quasi;
-        info->converged = r.converged;
-    }
-    return r.converged ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
+    return solve_mg(mg, false, mode, b, x, opts, k, info);
+}
+
+int helm3d_solve_mg_mixed(helm3d_mg* mg, int mode, const void* b, void* x, const helm3d_solve_opts* opts,
+                          const int* k, helm3d_solve_info* info)
+{
+    return solve_mg(mg, true, mode, b, x, opts, k, info);
 }
 
 }  // extern "C"

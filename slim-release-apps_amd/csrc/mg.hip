@@ -4,7 +4,8 @@
 // LinInterp1D.m), restriction = 1/8 times its transpose.  Per axis the host
 // supplies, for every fine index, the coarse bracket I and the weights of I
 // and I+1, and for every coarse index the list of fine indices that use it
-// (row-compressed), so both directions are gathers.
+// (row-compressed), so both directions are gathers.  Sums are formed in FP64
+// for complex128 and complex64 vectors alike.
 #include "mg.hpp"
 
 namespace helm3d {
@@ -13,8 +14,9 @@ namespace {
 
 constexpr int KB = 256;
 
-__global__ __launch_bounds__(KB) void prolong_add_kernel(MgAxes T, const cpx<double>* __restrict__ xc,
-                                                         cpx<double>* __restrict__ xf)
+template <typename R>
+__global__ __launch_bounds__(KB) void prolong_add_kernel(MgAxes T, const cpx<R>* __restrict__ xc,
+                                                         cpx<R>* __restrict__ xf)
 {
     const long nfx = T.nf[0], nfy = T.nf[1], nfz = T.nf[2];
     const long N = nfx * nfy * nfz;
@@ -36,19 +38,20 @@ __global__ __launch_bounds__(KB) void prolong_add_kernel(MgAxes T, const cpx<dou
 #pragma unroll
                 for (int a = 0; a < 2; ++a) {
                     const double w = wx[a] * wy[b] * wz[c];
-                    const cpx<double> v = xc[jx[a] + jy[b] * ncx + jz[c] * ncxy];
-                    re += w * v.re;
-                    im += w * v.im;
+                    const cpx<R> v = xc[jx[a] + jy[b] * ncx + jz[c] * ncxy];
+                    re += w * double(v.re);
+                    im += w * double(v.im);
                 }
-        cpx<double> o = xf[p];
-        o.re += re;
-        o.im += im;
+        cpx<R> o = xf[p];
+        o.re = R(double(o.re) + re);
+        o.im = R(double(o.im) + im);
         xf[p] = o;
     }
 }
 
-__global__ __launch_bounds__(KB) void restrict_kernel(MgAxes T, double scale, const cpx<double>* __restrict__ rf,
-                                                      cpx<double>* __restrict__ yc)
+template <typename R>
+__global__ __launch_bounds__(KB) void restrict_kernel(MgAxes T, double scale, const cpx<R>* __restrict__ rf,
+                                                      cpx<R>* __restrict__ yc)
 {
     const long ncx = T.nc[0], ncy = T.nc[1], ncz = T.nc[2];
     const long N = ncx * ncy * ncz;
@@ -64,13 +67,13 @@ __global__ __launch_bounds__(KB) void restrict_kernel(MgAxes T, double scale, co
                 const double wyz = wz * T.ax[1].fw[ey];
                 for (int ex = T.ax[0].rp[i]; ex < T.ax[0].rp[i + 1]; ++ex) {
                     const double w = wyz * T.ax[0].fw[ex];
-                    const cpx<double> v = rf[T.ax[0].fi[ex] + fy * nfx + fz * nfxy];
-                    re += w * v.re;
-                    im += w * v.im;
+                    const cpx<R> v = rf[T.ax[0].fi[ex] + fy * nfx + fz * nfxy];
+                    re += w * double(v.re);
+                    im += w * double(v.im);
                 }
             }
         }
-        yc[q] = cpx<double>{scale * re, scale * im};
+        yc[q] = cpx<R>{R(scale * re), R(scale * im)};
     }
 }
 
@@ -78,18 +81,25 @@ int blocks_for(long n) { return int(std::min<long>(std::max<long>((n + KB - 1) /
 
 }  // namespace
 
-hipError_t launch_prolong_add(const MgAxes& T, const cpx<double>* xc, cpx<double>* xf, hipStream_t s)
+template <typename R>
+hipError_t launch_prolong_add(const MgAxes& T, const cpx<R>* xc, cpx<R>* xf, hipStream_t s)
 {
     const long N = long(T.nf[0]) * T.nf[1] * T.nf[2];
-    hipLaunchKernelGGL(prolong_add_kernel, dim3(blocks_for(N)), dim3(KB), 0, s, T, xc, xf);
+    hipLaunchKernelGGL(prolong_add_kernel<R>, dim3(blocks_for(N)), dim3(KB), 0, s, T, xc, xf);
     return hipGetLastError();
 }
 
-hipError_t launch_restrict(const MgAxes& T, double scale, const cpx<double>* rf, cpx<double>* yc, hipStream_t s)
+template <typename R>
+hipError_t launch_restrict(const MgAxes& T, double scale, const cpx<R>* rf, cpx<R>* yc, hipStream_t s)
 {
     const long N = long(T.nc[0]) * T.nc[1] * T.nc[2];
-    hipLaunchKernelGGL(restrict_kernel, dim3(blocks_for(N)), dim3(KB), 0, s, T, scale, rf, yc);
+    hipLaunchKernelGGL(restrict_kernel<R>, dim3(blocks_for(N)), dim3(KB), 0, s, T, scale, rf, yc);
     return hipGetLastError();
 }
+
+template hipError_t launch_prolong_add<double>(const MgAxes&, const cpx<double>*, cpx<double>*, hipStream_t);
+template hipError_t launch_prolong_add<float>(const MgAxes&, const cpx<float>*, cpx<float>*, hipStream_t);
+template hipError_t launch_restrict<double>(const MgAxes&, double, const cpx<double>*, cpx<double>*, hipStream_t);
+template hipError_t launch_restrict<float>(const MgAxes&, double, const cpx<float>*, cpx<float>*, hipStream_t);
 
 }  // namespace helm3d

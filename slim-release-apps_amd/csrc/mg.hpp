@@ -23,9 +23,11 @@ struct MgAxes {
     MgAxis ax[3];
 };
 
-// xf += P xc
-hipError_t launch_prolong_add(const MgAxes& T, const cpx<double>* xc, cpx<double>* xf, hipStream_t s);
+// xf += P xc (R = double or float)
+template <typename R>
+hipError_t launch_prolong_add(const MgAxes& T, const cpx<R>* xc, cpx<R>* xf, hipStream_t s);
 // yc = scale * P^T rf
-hipError_t launch_restrict(const MgAxes& T, double scale, const cpx<double>* rf, cpx<double>* yc, hipStream_t s);
+template <typename R>
+hipError_t launch_restrict(const MgAxes& T, double scale, const cpx<R>* rf, cpx<R>* yc, hipStream_t s);
 
 }  // namespace helm3d

@@ -131,6 +131,8 @@ _SIGS = [
     ("helm3d_mg_transfer", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("helm3d_solve_mg", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts), _ip,
                                        ctypes.POINTER(SolveInfo)]),
+    ("helm3d_solve_mg_mixed", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts), _ip,
+                                             ctypes.POINTER(SolveInfo)]),
     ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
     ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),

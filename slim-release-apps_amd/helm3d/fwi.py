@@ -78,9 +78,12 @@ class FWIModel:
 class FWIOpts:
     """PDEopts.m:108-127 and LinSolveOpts.m:55-62 fields used on this path.
 
-    solver: "fgmres" (FGMRES.m in complex128) or "mixed" (FP64 flexible GMRES
-    with a complex64 BiCGStab preconditioner); tol is the relative residual of
-    every solve.  autotune picks the stencil launch shape per frequency
+    solver: "fgmres" (FGMRES.m in complex128), "mixed" (FP64 flexible GMRES
+    with a complex64 BiCGStab preconditioner) or "mg" (FGMRES preconditioned
+    by the ML_GMRES V-cycle, mg_levels levels, mg_k = ML_GMRES.m's
+    (ks_outer, ks_inner, kc_outer, kc_inner), the V-cycle in complex128 or,
+    with mg_precision "mixed", in complex64); tol is the relative residual
+    of every solve.  autotune picks the stencil launch shape per frequency
     (one-at-a-time solves).
     batch > 1 solves that many sources of a frequency together
     (helm3d_batch_solve: FGMRES_MT.m semantics for "fgmres", a fixed
@@ -101,6 +104,9 @@ class FWIOpts:
     batch: int = 1
     inner_iters: int = 20
     autotune: bool = False
+    mg_levels: int = 3
+    mg_k: tuple = (3, 5, 3, 5)
+    mg_precision: str = "c128"
     stats: list = field(default_factory=list, repr=False)
 
 
@@ -259,6 +265,7 @@ class FWIProblem:
             raise ValueError("size(Q,1) must match the source grid")
         self.H = Helm3d(device=self.device)
         self._batch = None
+        self._mg = None   # multigrid hierarchy of the current frequency (solver "mg")
         self._geom = {}   # npml key -> (Ps, Pr, Pr^T)
 
     # -- grids and operators --------------------------------------------------
@@ -291,6 +298,15 @@ class FWIProblem:
         if self.opts.autotune and self.opts.batch <= 1:
             # launch shape of the stencil the one-at-a-time solves spend their time in
             self.H.autotune(mode=1, precision="c64" if self.opts.solver == "mixed" else "c128", reps=5)
+        if self.opts.solver == "mg":
+            key = (nt, p6, self.opts.mg_levels, tuple(self.opts.mg_k))
+            if self._mg is None or self._mg[0] != key:
+                from .mg import MLGMRES
+                self._mg = None
+                self._mg = (key, MLGMRES(v, self.n, self.d, int(npml[0, 0]), freq, m.unit, self.opts.mg_levels,
+                                         self.opts.mg_k, self.device, free_surface=self.opts.free_surface))
+            else:
+                self._mg[1].set_model(v, freq)
         Ps, Pr, PrT = self._geometry(npml)
         return dict(npml=npml, p6=p6, nt=nt, N=int(np.prod(nt)), wn=wn, dwn=dwn, Ps=Ps, Pr=Pr, PrT=PrT)
 
@@ -325,15 +341,18 @@ class FWIProblem:
         elif o.solver == "mixed":
             x, info = self.H.solve_mixed(b, mode=mode, tol=o.tol, maxinnerit=o.maxinnerit, maxit=o.maxit,
                                          inner_tol=o.inner_tol, inner_maxit=o.inner_maxit)
+        elif o.solver == "mg":
+            x, info = self._mg[1].solve(b, mode=mode, tol=o.tol, maxinnerit=o.maxinnerit, maxit=o.maxit,
+                                        precision=o.mg_precision)
         else:
-            raise ValueError("solver must be 'fgmres' or 'mixed'")
+            raise ValueError("solver must be 'fgmres', 'mixed' or 'mg'")
         o.stats.append(info)
         return x
 
     def _solve_many(self, bs, mode: int):
         """Solves for a list of right-hand sides, opts.batch at a time."""
         o = self.opts
-        if o.batch <= 1:
+        if o.batch <= 1 or o.solver == "mg":
             return [self._solve(b, mode) for b in bs]
         from .batch import BatchSolver
         if self._batch is None:

@@ -24,9 +24,16 @@ from .core import Helm3d, SolveInfo, _stream_ptr
 from .fwi import lagrange_cubic
 
 
-def coarsen_model(v: np.ndarray, n, n_coarse) -> np.ndarray:
+def coarsen_model(v, n, n_coarse):
     """to_coarse * vec(v) of fine2coarse(n, d, 2d) with cubic interpolation
-    (fine2coarse.m:28-53): separable, x fastest."""
+    (fine2coarse.m:28-53): separable, x fastest.  v: numpy array (host) or
+    torch tensor (the product runs it on the model's device)."""
+    if isinstance(v, torch.Tensor):
+        V = v.reshape(n[2], n[1], n[0]).to(torch.float64)
+        for ax, a in ((2, 0), (1, 1), (0, 2)):
+            W = torch.from_numpy(lagrange_cubic(np.linspace(0, 1, n[a]), np.linspace(0, 1, n_coarse[a])))
+            V = torch.movedim(torch.tensordot(W.to(V.device), V, dims=([1], [ax])), 0, ax)
+        return V.reshape(-1).contiguous()
     V = np.asarray(v, dtype=np.float64).reshape(n[2], n[1], n[0])
     for ax, a in ((2, 0), (1, 1), (0, 2)):  # numpy axis <-> grid axis a
         W = lagrange_cubic(np.linspace(0, 1, n[a]), np.linspace(0, 1, n_coarse[a]))
@@ -35,36 +42,34 @@ def coarsen_model(v: np.ndarray, n, n_coarse) -> np.ndarray:
 
 
 class MLGMRES:
-    """Multigrid hierarchy for one model and frequency, and its solver.
+    """Multigrid hierarchy for one model grid, and its solver.
 
-    v: physical model (prod(n_phys) values, x fastest) in ``unit``;
+    v: physical model (prod(n_phys) values, x fastest) in ``unit``, a numpy
+    array or a device tensor; npml: PML points of level 0 on every face
+    (construct_multigrid.m:21 takes comp_grid.pml(1,1)); free_surface drops
+    the top (z lo) PML on every level, as discrete_helmholtz does;
     k = (ks_outer, ks_inner, kc_outer, kc_inner) of ML_GMRES.m:3-13."""
 
     def __init__(self, v, n_phys, h, npml: int, freq: float, unit: str = "m/s", nlevels: int = 3,
-                 k: Sequence[int] = (3, 5, 3, 5), device=None):
+                 k: Sequence[int] = (3, 5, 3, 5), device=None, free_surface: bool = False):
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.device = device
         self.k = tuple(int(a) for a in k)
+        self.unit = unit
         n = tuple(int(a) for a in n_phys)
         h = tuple(float(a) for a in np.asarray(h, dtype=np.float64).reshape(-1)) if np.size(h) == 3 else (float(h),) * 3
-        vl = np.asarray(v, dtype=np.float64).reshape(-1)
         self.ops, self.smoothers, self.grids = [], [], []
         for lvl in range(nlevels):
             if lvl > 0:
-                nc = tuple(int(math.ceil(a / 2)) for a in n)
-                vl = coarsen_model(vl, n, nc)
-                n = nc
+                n = tuple(int(math.ceil(a / 2)) for a in n)
                 h = tuple(2.0 * a for a in h)
-            p = int(math.ceil(npml / 2 ** lvl))
-            p6 = (p,) * 6
-            A = Helm3d(device=device)
-            wn, _, _ = A.param2wavenum(torch.from_numpy(vl).to(device), n, p6, freq, unit, derivs=0)
-            nt = tuple(a + 2 * p for a in n)
-            A.set_model(nt, h, p6, wn)
-            S = Helm3d(nt, h, p6, wn)
-            self.ops.append(A)
-            self.smoothers.append(S)
-            self.grids.append(dict(n_phys=n, nt=nt, h=h, npml=p))
+            p = int(math.ceil(int(npml) / 2 ** lvl))
+            p6 = (p, p, p, p, 0 if free_surface else p, p)
+            nt = tuple(n[a] + p6[2 * a] + p6[2 * a + 1] for a in range(3))
+            self.grids.append(dict(n_phys=n, nt=nt, h=h, npml=p, p6=p6))
+            self.ops.append(Helm3d(device=device))
+            self.smoothers.append(Helm3d(device=device))
+        self.set_model(v, freq)
         self._mg = ctypes.c_void_p()
         L = len(self.ops)
         a_arr = (ctypes.c_void_p * L)(*[H._ctx.value for H in self.ops])
@@ -72,6 +77,21 @@ class MLGMRES:
         with torch.cuda.device(device):
             check(lib().helm3d_set_stream(self.ops[0]._ctx, ctypes.c_void_p(_stream_ptr())))
             check(lib().helm3d_mg_create(L, a_arr, s_arr, ctypes.byref(self._mg)))
+
+    def set_model(self, v, freq: float):
+        """Load the model of every level (construct_multigrid.m:46-49: cubic
+        coarsening of the previous level's physical model, discrete_helmholtz
+        on the coarse grid); grids and transfers stay as built."""
+        vl = v if isinstance(v, torch.Tensor) else torch.from_numpy(np.asarray(v, dtype=np.float64).reshape(-1))
+        vl = vl.reshape(-1).to(device=self.device, dtype=torch.float64)
+        for lvl, g in enumerate(self.grids):
+            if lvl > 0:
+                vl = coarsen_model(vl, self.grids[lvl - 1]["n_phys"], g["n_phys"])
+            A, S = self.ops[lvl], self.smoothers[lvl]
+            wn, _, _ = A.param2wavenum(vl, g["n_phys"], g["p6"], freq, self.unit, derivs=0)
+            A.set_model(g["nt"], g["h"], g["p6"], wn)
+            S.set_model(g["nt"], g["h"], g["p6"], wn)
+        self.freq = float(freq)
 
     @property
     def H(self) -> Helm3d:
@@ -104,8 +124,12 @@ class MLGMRES:
         return out
 
     def solve(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxinnerit: int = 20, maxit: int = 10000,
-              x0: Optional[torch.Tensor] = None, raise_on_notconv: bool = False):
-        """FGMRES on the level-0 operator preconditioned by the V-cycle."""
+              x0: Optional[torch.Tensor] = None, raise_on_notconv: bool = False, precision: str = "c128"):
+        """FGMRES on the level-0 operator preconditioned by the V-cycle;
+        precision "mixed" runs the V-cycle in complex64 (the outer FGMRES and
+        its residuals stay FP64)."""
+        if precision not in ("c128", "mixed"):
+            raise ValueError("precision must be 'c128' or 'mixed'")
         H = self.ops[0]
         b = H._vec(b, torch.complex128)
         x = torch.zeros_like(b) if x0 is None else H._vec(x0, torch.complex128).clone()
@@ -113,8 +137,9 @@ class MLGMRES:
         info = _lib.SolveInfo()
         kk = (ctypes.c_int * 4)(*self.k)
         with torch.cuda.device(self.device):
-            code = lib().helm3d_solve_mg(self._mg, int(mode), ctypes.c_void_p(b.data_ptr()),
-                                         ctypes.c_void_p(x.data_ptr()), ctypes.byref(o), kk, ctypes.byref(info))
+            fn = lib().helm3d_solve_mg if precision == "c128" else lib().helm3d_solve_mg_mixed
+            code = fn(self._mg, int(mode), ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                      ctypes.byref(o), kk, ctypes.byref(info))
         check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
         return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
                             bool(info.converged))
