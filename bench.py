@@ -201,7 +201,7 @@ def uneven_leg(plain_rhs_per_s):
             "dtype": "c64 inner / f64 outer"}
 
 
-def mg_leg(plain_rhs_per_s, solves=3):
+def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128"), (3, "mixed"), (4, "c128"), (4, "mixed"))):
     """The benchmark solve (240x240x140, 6 Hz, one impulse, 1e-6 FP64
     residual) with FGMRES(20) preconditioned by the ML_GMRES V-cycle: 3 levels
     (ML_GMRES.m) and 4 levels, the V-cycle in complex128 and in complex64."""
@@ -211,13 +211,13 @@ def mg_leg(plain_rhs_per_s, solves=3):
     cfg = config_c2()
     dev = torch.device("cuda:0")
     runs = []
-    for levels in (3, 4):
+    for levels in sorted({lv for lv, _ in configs}):
         M = MLGMRES(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=levels, device=dev)
         nx, ny, _ = M.grids[0]["nt"]
         p, s = cfg.npml, cfg.sources[0]
         b = torch.zeros(M.H.N, dtype=torch.complex128, device=dev)
         b[(s[0] + p) + (s[1] + p) * nx + (s[2] + p) * nx * ny] = 1.0
-        for prec in ("c128", "mixed"):
+        for prec in [p for lv, p in configs if lv == levels]:
             M.solve(b, tol=1e-6, precision=prec)  # warm-up (workspaces)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -331,6 +331,8 @@ def main():
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
+    ap.add_argument("--mg-config", default=None,
+                    help="--mode mg only: one 'levels,precision' pair (e.g. 4,mixed) instead of all four")
     ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg"), default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
                          "(profiling workload); copy: K device copies of 256 MiB (PMC calibration); "
@@ -358,7 +360,11 @@ def main():
         elif args.mode == "uneven":
             res = uneven_leg(plain_rhs_per_s)
         elif args.mode == "mg":
-            res = mg_leg(plain_rhs_per_s)
+            if args.mg_config:
+                lv, pr = args.mg_config.split(",")
+                res = mg_leg(plain_rhs_per_s, solves=args.steps, configs=((int(lv), pr),))
+            else:
+                res = mg_leg(plain_rhs_per_s)
         else:
             res = dict(zip(("grads_per_min", "fwi"), fwi_leg()))
         print(json.dumps({"mode": args.mode, **res}))

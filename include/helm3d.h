@@ -134,7 +134,8 @@ int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *
                        const void *x_dev, void *y_dev);
 
 /* Stencil launch shape: z planes per block (0: automatic), column-tile rows
- * (4 or 8: 256 or 512 threads) and planes of loads in flight (1 or 2).  The
+ * (4 or 8: 256 or 512 threads) and planes of loads in flight (1 to 4; the
+ * batched kernels use at most 2).  The
  * result is identical for every shape; only speed changes.  Chosen per grid by
  * the autotuner of the Python package (helm3d.tune). */
 int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch);

@@ -33,14 +33,14 @@ hipError_t launch_b(const StencilLaunch<R>* items, const int* first, int nitems,
 {
     const dim3 grid(blocks);
     if (ty == 8) {
-        if (pf == 2)
+        if (pf >= 2)
             hipLaunchKernelGGL((helm27_batch_kernel<R, A, false, W, 8, 2, EPI>), grid, dim3(TX * 8), 0, s, items,
                                first, nitems, active);
         else
             hipLaunchKernelGGL((helm27_batch_kernel<R, A, false, W, 8, 1, EPI>), grid, dim3(TX * 8), 0, s, items,
                                first, nitems, active);
     } else {
-        if (pf == 2)
+        if (pf >= 2)
             hipLaunchKernelGGL((helm27_batch_kernel<R, A, false, W, 4, 2, EPI>), grid, dim3(TX * 4), 0, s, items,
                                first, nitems, active);
         else

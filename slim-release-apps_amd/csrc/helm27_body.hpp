@@ -84,7 +84,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     constexpr int HY = TY + 2;
     constexpr int NS = HX * HY;
     static_assert(NS <= 2 * NT, "halo tile must fit two load slots per thread");
-    static_assert(PF == 1 || PF == 2, "prefetch depth 1 or 2");
+    static_assert(PF >= 1 && PF <= 4, "prefetch depth 1 to 4");
     const OpDesc<R>& op = L.op;
     __shared__ cpx<R> su[2][HY][HX];            // input vector plane (+halo)
     // forward mass operand: wn .* x (WNC, DERIV) or the real wn itself, the
@@ -92,7 +92,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     constexpr bool kWnSep = !ADJ && !WNC && !DERIV;
     __shared__ cpx<R> sm[(ADJ || kWnSep) ? 1 : 2][HY][HX];
     __shared__ R sw[kWnSep ? 2 : 1][HY][HX];
-    __shared__ cpx<R> szl[kMaxChunk + 4], szh[kMaxChunk + 4];  // z PML slots k0 .. k1+3
+    __shared__ cpx<R> szl[kMaxChunk + 2 + PF], szh[kMaxChunk + 2 + PF];  // z PML slots k0 .. k1+PF+1
 
     const int nx = op.nx, ny = op.ny, nz = op.nz;
 
@@ -138,8 +138,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             py0 = conjc(op.yhi[jc]); py2 = conjc(op.ylo[jc + 2]); pyf = conjc(op.ylo[jc + 1] + op.yhi[jc + 1]);
         }
         // z weights of this chunk: slots k0 .. k1+1 (planes k0-1 .. k1) plus
-        // two padding entries read by the trailing pipeline step
-        for (int q = tid; q < k1 - k0 + 4; q += NT) {
+        // the padding entries read by the trailing pipeline steps
+        for (int q = tid; q < k1 - k0 + 2 + PF; q += NT) {
             const int sl = min(k0 + q, nz + 1);
             szl[q] = op.zlo[sl];
             szh[q] = op.zhi[sl];
@@ -472,20 +472,20 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // padding step computes but emits nothing).  No memory operation in the
     // loop body is conditional on the step, so the compiler can keep PF sets
     // of loads in flight across the back-edge.
-    Set A{}, B{};
-    fetch(k0 - 1, A);
-    if constexpr (PF == 2) {
-        // keep set A's loads ahead of set B's: the loop then waits only for
-        // the set it stages (the wait count at the loop header is merged
-        // from this prologue and the back edge)
-        __builtin_amdgcn_sched_barrier(0);
-        fetch(k0, B);
+    Set S[PF];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+        // keep each set's loads ahead of the next set's: the loop then waits
+        // only for the set it stages (the wait count at the loop header is
+        // merged from this prologue and the back edge)
+        if (p > 0) __builtin_amdgcn_sched_barrier(0);
+        fetch(k0 - 1 + p, S[p]);
     }
     const int ntrips = (k1 - k0 + 2 + PF - 1) / PF;
     for (int tr = 0; tr < ntrips; ++tr) {
         const int q = k0 - 1 + tr * PF;
-        step(q, tr * PF, A);
-        if constexpr (PF == 2) step(q + 1, tr * PF + 1, B);
+#pragma unroll
+        for (int p = 0; p < PF; ++p) step(q + p, tr * PF + p, S[p]);
     }
 
     // optional per-block partials (deterministic two-stage reduction):

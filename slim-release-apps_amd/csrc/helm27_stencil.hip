@@ -18,10 +18,11 @@
 // The adjoint is the exact conjugate transpose: F/G take the neighbour's x/y
 // PML values and the z weights come from the neighbouring planes.
 //
-// Tiling.  A 256-thread block owns a 64 x 4 column tile and marches over a z
-// chunk.  Each plane (with a one-point x/y halo) is staged once in LDS
-// (double-buffered, one barrier per plane); the next plane's global loads are
-// issued before the barrier so HBM latency overlaps the current plane.  Block
+// Tiling.  A block of 64 x TY threads (TY = 4 or 8) owns a 64 x TY column tile
+// and marches over a z chunk.  Each plane (with a one-point x/y halo) is
+// staged once in LDS (double-buffered, one barrier per plane); the global
+// loads of the next PF planes (PF = 1..4) are in flight while a plane is
+// computed, so HBM latency overlaps the march.  Block
 // ids are remapped so consecutive tiles of the same chunk land on the same XCD
 // and share halo lines in its L2.
 #include "helm27_body.hpp"
@@ -43,7 +44,7 @@ StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty, int pf)
 {
     StencilGrid g;
     g.ty = (ty == 8) ? 8 : 4;
-    g.pf = (pf == 2) ? 2 : 1;
+    g.pf = std::max(1, std::min(pf, 4));
     g.tiles_x = (nx + TX - 1) / TX;
     g.tiles_y = (ny + g.ty - 1) / g.ty;
     if (kchunk <= 0) {
@@ -67,11 +68,19 @@ hipError_t launch_shape(const StencilLaunch<R>& L, const StencilGrid& g, hipStre
 {
     const dim3 grid(g.blocks);
     if (g.ty == 8) {
-        if (g.pf == 2) hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 2, EPI>), grid, dim3(TX * 8), 0, s, L);
-        else hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 1, EPI>), grid, dim3(TX * 8), 0, s, L);
+        switch (g.pf) {
+        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 1, EPI>), grid, dim3(TX * 8), 0, s, L); break;
+        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 2, EPI>), grid, dim3(TX * 8), 0, s, L); break;
+        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 3, EPI>), grid, dim3(TX * 8), 0, s, L); break;
+        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 4, EPI>), grid, dim3(TX * 8), 0, s, L); break;
+        }
     } else {
-        if (g.pf == 2) hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 2, EPI>), grid, dim3(TX * 4), 0, s, L);
-        else hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 1, EPI>), grid, dim3(TX * 4), 0, s, L);
+        switch (g.pf) {
+        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 1, EPI>), grid, dim3(TX * 4), 0, s, L); break;
+        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 2, EPI>), grid, dim3(TX * 4), 0, s, L); break;
+        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 3, EPI>), grid, dim3(TX * 4), 0, s, L); break;
+        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 4, EPI>), grid, dim3(TX * 4), 0, s, L); break;
+        }
     }
     return hipGetLastError();
 }

@@ -659,8 +659,8 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
 int helm3d_set_launch(helm3d_ctx* c, int kchunk, int tile_rows, int prefetch)
 {
     CHK(check_model(c));
-    if (!(tile_rows == 4 || tile_rows == 8) || !(prefetch == 1 || prefetch == 2) || kchunk < 0)
-        return fail(HELM3D_ERR_ARG, "tile_rows in {4,8}, prefetch in {1,2}, kchunk >= 0");
+    if (!(tile_rows == 4 || tile_rows == 8) || prefetch < 1 || prefetch > 4 || kchunk < 0)
+        return fail(HELM3D_ERR_ARG, "tile_rows in {4,8}, prefetch in 1..4, kchunk >= 0");
     c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kchunk, tile_rows, prefetch);
     CHK(ensure_work(c));
     c->inner_graph_iters = -1;

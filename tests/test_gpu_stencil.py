@@ -114,7 +114,7 @@ def test_device_deriv():
     assert relerr(ya, O.hmvp("adj_deriv", dwn, h, n, np.array(npml), x)) <= 1e-12
 
 
-@pytest.mark.parametrize("shape", [(0, 4, 1), (35, 4, 2), (70, 8, 1), (17, 8, 2)])
+@pytest.mark.parametrize("shape", [(0, 4, 1), (35, 4, 2), (70, 8, 1), (17, 8, 2), (23, 4, 3), (9, 8, 4), (70, 4, 4)])
 def test_launch_shapes_identical(shape):
     """Every launch shape gives bit-identical results (interior and PML waves)."""
     import torch
