@@ -115,7 +115,9 @@ int helm3d_set_stream(helm3d_ctx *ctx, void *stream);
  * counts and wn on the PML grid.  wn_dev points to DEVICE memory: N doubles
  * when wn_complex == 0, N interleaved complex doubles when wn_complex != 0
  * (complex frequency / WNCMPLX).  The context keeps its own FP64 and FP32
- * copies, so the caller may free wn_dev afterwards. */
+ * copies, so the caller may free wn_dev afterwards.  A model on the same grid
+ * as the previous one keeps the stencil launch shape (helm3d_set_launch,
+ * helm3d_autotune); a new grid starts from the default shape. */
 int helm3d_set_model(helm3d_ctx *ctx, const int n[3], const double h[3], const int npml[6],
                      const void *wn_dev, int wn_complex);
 

@@ -543,6 +543,8 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
         if (npml[a] < 0) return fail(HELM3D_ERR_ARG, "npml must be non-negative");
     const long N = long(n[0]) * n[1] * n[2];
     if (N >= (1L << 40)) return fail(HELM3D_ERR_ARG, "grid too large");
+    // a new model on the same grid keeps the launch shape (tuned or chosen)
+    const bool same_grid = c->has_model && c->n[0] == n[0] && c->n[1] == n[1] && c->n[2] == n[2];
     std::memcpy(c->n, n, sizeof(c->n));
     std::memcpy(c->npml, npml, sizeof(c->npml));
     std::memcpy(c->h, h, sizeof(c->h));
@@ -648,7 +650,7 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
     };
     fill(c->op64, c->tab64, c->wn64);
     fill(c->op32, c->tab32, c->wn32);
-    c->grid = stencil_grid<double>(n[0], n[1], n[2], 0);
+    if (!same_grid) c->grid = stencil_grid<double>(n[0], n[1], n[2], 0);
     CHK(ensure_work(c));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_model = true;

@@ -48,10 +48,14 @@ class MLGMRES:
     array or a device tensor; npml: PML points of level 0 on every face
     (construct_multigrid.m:21 takes comp_grid.pml(1,1)); free_surface drops
     the top (z lo) PML on every level, as discrete_helmholtz does;
-    k = (ks_outer, ks_inner, kc_outer, kc_inner) of ML_GMRES.m:3-13."""
+    k = (ks_outer, ks_inner, kc_outer, kc_inner) of ML_GMRES.m:3-13.
+    autotune: time the stencil launch shapes of every level once (the FP64
+    shape for the level-0 operator, the complex64 shape for the smoothers and
+    coarse levels); shapes survive set_model and never change results."""
 
     def __init__(self, v, n_phys, h, npml: int, freq: float, unit: str = "m/s", nlevels: int = 3,
-                 k: Sequence[int] = (3, 5, 3, 5), device=None, free_surface: bool = False):
+                 k: Sequence[int] = (3, 5, 3, 5), device=None, free_surface: bool = False,
+                 autotune: bool = True):
         device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.device = device
         self.k = tuple(int(a) for a in k)
@@ -70,6 +74,10 @@ class MLGMRES:
             self.ops.append(Helm3d(device=device))
             self.smoothers.append(Helm3d(device=device))
         self.set_model(v, freq)
+        if autotune:
+            for lvl in range(nlevels):
+                self.ops[lvl].autotune(mode=1, precision="c128" if lvl == 0 else "c64", reps=5)
+                self.smoothers[lvl].autotune(mode=1, precision="c64", reps=5)
         self._mg = ctypes.c_void_p()
         L = len(self.ops)
         a_arr = (ctypes.c_void_p * L)(*[H._ctx.value for H in self.ops])
