@@ -186,6 +186,25 @@ def uneven_leg(plain_rhs_per_s):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     rate = len(bs) / dt
+    del xs
+    torch.cuda.empty_cache()
+    # the same right-hand sides with FGMRES + 4-level complex64 V-cycle, four
+    # hierarchies per subproblem (set-up per subproblem not timed)
+    from helm3d.mg import MGPool
+    mg_t, mg_conv, mg_rel = 0.0, True, 0.0
+    for k, c in enumerate(subs):
+        pool = MGPool(vec(c.v), c.n_phys, c.h, c.npml, c.freq, nlevels=4, device=dev, workers=4)
+        mine = [bs[i] for i in range(len(bs)) if owner[i] == k]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, inf = pool.solve_many(mine, precision="mixed")
+        torch.cuda.synchronize()
+        mg_t += time.perf_counter() - t1
+        mg_conv = mg_conv and all(i.converged for i in inf)
+        mg_rel = max(mg_rel, max(i.relres for i in inf))
+        pool.close()
+        del pool
+        torch.cuda.empty_cache()
     per_sub = []
     for k, c in enumerate(subs):
         its = [infos[i].iters for i in range(len(bs)) if owner[i] == k]
@@ -198,7 +217,9 @@ def uneven_leg(plain_rhs_per_s):
             "grid_point_rhs_per_s": sum(H.N for H in ops) / dt,
             "batches": [len(ch) for ch in chunks], "converged": all(i.converged for i in infos),
             "max_relres": max(i.relres for i in infos), "subproblems": per_sub, "launch": tuned,
-            "dtype": "c64 inner / f64 outer"}
+            "dtype": "c64 inner / f64 outer",
+            "multigrid": {"rhs_per_s": len(bs) / mg_t, "seconds": mg_t, "converged": mg_conv, "max_relres": mg_rel,
+                          "solver": "FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, 4 streams per subproblem"}}
 
 
 def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128"), (3, "mixed"), (4, "c128"), (4, "mixed"))):
@@ -241,18 +262,56 @@ def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128"), (3, "mixed"), (4, "c
             "dtype": "f64 outer / c128 or c64 V-cycle"}
 
 
+def mg_sources_leg(plain_rhs_per_s, workers=4):
+    """The 8 surface sources of the mixed-precision workload (C2 model) solved
+    to 1e-6 FP64 residual by FGMRES with a 4-level complex64 V-cycle, four
+    hierarchies on four streams; error of each solution against a complex128
+    solve to 1e-10."""
+    import torch
+    from helm3d.mg import MGPool
+    from helm3d.models import config_c3, vec
+    cfg = config_c3()
+    dev = torch.device("cuda:0")
+    pool = MGPool(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=4, device=dev, workers=workers)
+    nx, ny, _ = pool.grids[0]["nt"]
+    p = cfg.npml
+    bs = []
+    for s in cfg.sources:
+        b = torch.zeros(pool.H.N, dtype=torch.complex128, device=dev)
+        b[(s[0] + p) + (s[1] + p) * nx + (s[2] + p) * nx * ny] = 1.0
+        bs.append(b)
+    pool.solve_many(bs[:workers], precision="mixed")  # warm-up (workspaces)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs, infos = pool.solve_many(bs, precision="mixed")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    xr, _ = pool.solve_many(bs, tol=1e-10, precision="c128")
+    errs = [float((x - r).norm() / r.norm()) for x, r in zip(xs, xr)]
+    rel = [float((b - pool.H.apply(x)).norm() / b.norm()) for b, x in zip(bs, xs)]
+    rate = len(bs) / dt
+    pool.close()
+    return {"workload": "8 point sources (z=5, y=100, x=round(linspace(20,179,8))) on the 240x240x140 benchmark "
+                        "model, FGMRES(20) in FP64 + 4-level complex64 ML_GMRES V-cycle (inner FGMRES smoothers "
+                        f"and coarse solves in complex64), {workers} hierarchies on {workers} streams, 1e-6 FP64 residual",
+            "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s if plain_rhs_per_s else None,
+            "outer_iters": [i.iters for i in infos], "final_relres": rel, "err_vs_complex128": errs,
+            "dtype": "c64 V-cycle / f64 outer"}
+
+
 def fwi_leg(solver="mg"):
     """One least-squares FWI gradient on the 256x256x128 workload; every
-    solve to 1e-6 with FGMRES preconditioned by a 4-level complex64 V-cycle
-    (solver "mg"), or with batched mixed-precision solves ("mixed")."""
+    solve to 1e-6 with FGMRES preconditioned by a 4-level complex64 V-cycle,
+    four solves at a time on four streams (solver "mg"), or with batched
+    mixed-precision solves ("mixed")."""
     import torch
     from helm3d.fwi import FWIOpts, FWIProblem
     from helm3d.models import fwi_c5
     model, m_true, m0 = fwi_c5()
     dev = torch.device("cuda:0")
     if solver == "mg":
-        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed")
-        how = "FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, FP64 outer"
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=4)
+        how = "FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, FP64 outer, 4 solves at a time on 4 streams"
     else:
         opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20)
         how = "batched mixed-precision solves (16 sources per batch)"
@@ -333,10 +392,11 @@ def main():
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
     ap.add_argument("--mg-config", default=None,
                     help="--mode mg only: one 'levels,precision' pair (e.g. 4,mixed) instead of all four")
-    ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg"), default="solve",
+    ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg", "mg-sources"),
+                    default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
                          "(profiling workload); copy: K device copies of 256 MiB (PMC calibration); "
-                         "mixed / uneven / fwi / mg: one of the side legs alone (profiling workloads)")
+                         "mixed / uneven / fwi / mg / mg-sources: one of the side legs alone (profiling workloads)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -353,12 +413,14 @@ def main():
 
     if args.mode == "copy":
         return copy_mode(args)
-    if args.mode in ("mixed", "uneven", "fwi", "mg"):
+    if args.mode in ("mixed", "uneven", "fwi", "mg", "mg-sources"):
         plain_rhs_per_s = None  # no plain solve in this run: ratios are left out
         if args.mode == "mixed":
             res = mixed_leg(plain_rhs_per_s)
         elif args.mode == "uneven":
             res = uneven_leg(plain_rhs_per_s)
+        elif args.mode == "mg-sources":
+            res = mg_sources_leg(plain_rhs_per_s)
         elif args.mode == "mg":
             if args.mg_config:
                 lv, pr = args.mg_config.split(",")
@@ -467,6 +529,10 @@ def main():
     if not args.no_mg:
         out["multigrid"] = mg_leg(args.steps / elapsed)
         print(f"[bench] multigrid leg done: {out['multigrid']['rhs_per_s']:.3f} RHS/s", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
+        out["multigrid_sources"] = mg_sources_leg(args.steps / elapsed)
+        print(f"[bench] multigrid sources leg done: {out['multigrid_sources']['rhs_per_s']:.3f} RHS/s",
+              file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_uneven:
         out["batched_uneven"] = uneven_leg(args.steps / elapsed)

@@ -82,8 +82,9 @@ class FWIOpts:
     with a complex64 BiCGStab preconditioner) or "mg" (FGMRES preconditioned
     by the ML_GMRES V-cycle, mg_levels levels, mg_k = ML_GMRES.m's
     (ks_outer, ks_inner, kc_outer, kc_inner), the V-cycle in complex128 or,
-    with mg_precision "mixed", in complex64); tol is the relative residual
-    of every solve.  autotune picks the stencil launch shape per frequency
+    with mg_precision "mixed", in complex64; mg_workers > 1 solves that many
+    right-hand sides at once on separate streams); tol is the relative
+    residual of every solve.  autotune picks the stencil launch shape per frequency
     (one-at-a-time solves).
     batch > 1 solves that many sources of a frequency together
     (helm3d_batch_solve: FGMRES_MT.m semantics for "fgmres", a fixed
@@ -107,6 +108,7 @@ class FWIOpts:
     mg_levels: int = 3
     mg_k: tuple = (3, 5, 3, 5)
     mg_precision: str = "c128"
+    mg_workers: int = 1
     stats: list = field(default_factory=list, repr=False)
 
 
@@ -299,12 +301,13 @@ class FWIProblem:
             # launch shape of the stencil the one-at-a-time solves spend their time in
             self.H.autotune(mode=1, precision="c64" if self.opts.solver == "mixed" else "c128", reps=5)
         if self.opts.solver == "mg":
-            key = (nt, p6, self.opts.mg_levels, tuple(self.opts.mg_k))
+            key = (nt, p6, self.opts.mg_levels, tuple(self.opts.mg_k), self.opts.mg_workers)
             if self._mg is None or self._mg[0] != key:
-                from .mg import MLGMRES
+                from .mg import MGPool
                 self._mg = None
-                self._mg = (key, MLGMRES(v, self.n, self.d, int(npml[0, 0]), freq, m.unit, self.opts.mg_levels,
-                                         self.opts.mg_k, self.device, free_surface=self.opts.free_surface))
+                self._mg = (key, MGPool(v, self.n, self.d, int(npml[0, 0]), freq, m.unit, self.opts.mg_levels,
+                                        self.opts.mg_k, self.device, free_surface=self.opts.free_surface,
+                                        workers=self.opts.mg_workers))
             else:
                 self._mg[1].set_model(v, freq)
         Ps, Pr, PrT = self._geometry(npml)
@@ -342,8 +345,9 @@ class FWIProblem:
             x, info = self.H.solve_mixed(b, mode=mode, tol=o.tol, maxinnerit=o.maxinnerit, maxit=o.maxit,
                                          inner_tol=o.inner_tol, inner_maxit=o.inner_maxit)
         elif o.solver == "mg":
-            x, info = self._mg[1].solve(b, mode=mode, tol=o.tol, maxinnerit=o.maxinnerit, maxit=o.maxit,
-                                        precision=o.mg_precision)
+            xs, infos = self._mg[1].solve_many([b], mode=mode, tol=o.tol, maxinnerit=o.maxinnerit, maxit=o.maxit,
+                                               precision=o.mg_precision)
+            x, info = xs[0], infos[0]
         else:
             raise ValueError("solver must be 'fgmres', 'mixed' or 'mg'")
         o.stats.append(info)
@@ -352,7 +356,12 @@ class FWIProblem:
     def _solve_many(self, bs, mode: int):
         """Solves for a list of right-hand sides, opts.batch at a time."""
         o = self.opts
-        if o.batch <= 1 or o.solver == "mg":
+        if o.solver == "mg":
+            xs, infos = self._mg[1].solve_many(bs, mode=mode, tol=o.tol, maxinnerit=o.maxinnerit, maxit=o.maxit,
+                                               precision=o.mg_precision)
+            o.stats.extend(infos)
+            return xs
+        if o.batch <= 1:
             return [self._solve(b, mode) for b in bs]
         from .batch import BatchSolver
         if self._batch is None:

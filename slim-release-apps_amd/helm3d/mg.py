@@ -151,3 +151,90 @@ class MLGMRES:
         check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
         return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
                             bool(info.converged))
+
+
+class MGPool:
+    """Several multigrid hierarchies of one model, each on its own stream and
+    host thread, so that the solves of many right-hand sides overlap on the
+    device (the coarse levels of one V-cycle leave most of the GPU idle).
+    Arguments as for MLGMRES, plus ``workers`` (hierarchies)."""
+
+    def __init__(self, v, n_phys, h, npml: int, freq: float, unit: str = "m/s", nlevels: int = 3,
+                 k: Sequence[int] = (3, 5, 3, 5), device=None, free_surface: bool = False,
+                 autotune: bool = True, workers: int = 2):
+        device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.device = device
+        self.streams = [torch.cuda.Stream(device) for _ in range(max(1, int(workers)))]
+        self.hier = []
+        for s in self.streams:
+            with torch.cuda.device(device), torch.cuda.stream(s):
+                self.hier.append(MLGMRES(v, n_phys, h, npml, freq, unit, nlevels, k, device, free_surface,
+                                         autotune))
+        for s in self.streams:
+            s.synchronize()
+
+    @property
+    def H(self) -> Helm3d:
+        return self.hier[0].H
+
+    @property
+    def grids(self):
+        return self.hier[0].grids
+
+    def set_model(self, v, freq: float):
+        for s, M in zip(self.streams, self.hier):
+            with torch.cuda.device(self.device), torch.cuda.stream(s):
+                M.set_model(v, freq)
+        for s in self.streams:
+            s.synchronize()
+
+    def close(self):
+        for M in self.hier:
+            M.close()
+        self.hier = []
+
+    def solve_many(self, bs: Sequence[torch.Tensor], mode: int = 1, tol: float = 1e-6, maxinnerit: int = 20,
+                   maxit: int = 10000, precision: str = "c128"):
+        """x_i = A^-1 b_i (A = H, or H^H when mode != 1) for every b_i, the
+        right-hand sides taken in turn by the hierarchies.  Returns
+        (xs, infos) in the order of bs; every x is ready on the caller's
+        current stream."""
+        import threading
+        main = torch.cuda.current_stream(self.device)
+        ready = torch.cuda.Event()
+        ready.record(main)
+        out: list = [None] * len(bs)
+        lock = threading.Lock()
+        nxt = [0]
+        errors: list = []
+
+        def work(w: int):
+            s, M = self.streams[w], self.hier[w]
+            try:
+                with torch.cuda.device(self.device), torch.cuda.stream(s):
+                    s.wait_event(ready)
+                    while True:
+                        with lock:
+                            i = nxt[0]
+                            nxt[0] += 1
+                        if i >= len(bs) or errors:
+                            break
+                        out[i] = M.solve(bs[i], mode=mode, tol=tol, maxinnerit=maxinnerit, maxit=maxit,
+                                         precision=precision)
+            except BaseException as e:  # re-raised on the calling thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=work, args=(w,)) for w in range(len(self.hier))]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for s in self.streams:
+            done = torch.cuda.Event()
+            done.record(s)
+            main.wait_event(done)
+        if errors:
+            raise errors[0]
+        for x, _ in out:
+            x.record_stream(main)
+        return [x for x, _ in out], [inf for _, inf in out]

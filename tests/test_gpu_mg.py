@@ -62,3 +62,24 @@ def test_mg_solve(mode, precision):
     assert info.converged and rel < 1e-6 and abs(rel - info.relres) < 1e-9
     assert info.iters * 5 < io["iters"]
     assert err < 1e-4
+
+
+def test_pool_of_hierarchies_matches_single():
+    """Right-hand sides solved three at a time on separate streams (MGPool)
+    give the solutions of one-at-a-time solves (to rounding: the hierarchies'
+    launch shapes are tuned separately, which can reorder the block sums of a
+    restart's residual norm)."""
+    import torch
+    from helm3d.mg import MGPool, MLGMRES
+    from helm3d.models import point_source, vec
+    cfg = _model()
+    bs = [torch.from_numpy(point_source(cfg, s)).cuda() for s in [(5, 5, 3), (20, 18, 3), (30, 10, 20), (12, 30, 8),
+                                                                  (33, 4, 3)]]
+    M = MLGMRES(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=3)
+    ref = [M.solve(b, tol=1e-6, precision="mixed") for b in bs]
+    pool = MGPool(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=3, workers=3)
+    xs, infos = pool.solve_many(bs, tol=1e-6, precision="mixed")
+    print("pool iterations", [i.iters for i in infos], flush=True)
+    for (xr, ir), x, inf in zip(ref, xs, infos):
+        assert inf.converged and inf.iters == ir.iters
+        assert float((x - xr).norm() / xr.norm()) <= 1e-12
