@@ -26,18 +26,18 @@ namespace {
 
 using kb::KB;
 
-// d_c = sum conj(v_c) w, c < K.
-template <typename R, int K>
+// d_c = sum conj(v_c) w, c < K.  NT: streaming (non-temporal) basis loads.
+template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                   const cpx<R>* __restrict__ w, long n,
                                                   double* __restrict__ part)
 {
-    kb::dots_body<R, K>(V, ldv, w, n, part, blockIdx.x, gridDim.x);
+    kb::dots_body<R, K, NT>(V, ldv, w, n, part, blockIdx.x, gridDim.x);
 }
 
 // w <- w - sum_{c<K} a_c v_c with a_c = scl_c * d_c (d: reduced dots in device
 // memory, so the host need not see them before this pass); partial ||w||^2.
-template <typename R, int K>
+template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                     const double* __restrict__ d, CoefPack scl,
                                                     cpx<R>* __restrict__ w, long n, double* __restrict__ part)
@@ -48,11 +48,11 @@ __global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V
         ar[c] = scl.v[c] * d[2 * c];
         ai[c] = scl.v[c] * d[2 * c + 1];
     }
-    kb::update_body<R, K>(V, ldv, ar, ai, w, n, part, blockIdx.x, gridDim.x);
+    kb::update_body<R, K, NT>(V, ldv, ar, ai, w, n, part, blockIdx.x, gridDim.x);
 }
 
 // x <- x + sum_{c<K} a_c v_c (the solution update of FGMRES.m:115-119).
-template <typename R, int K>
+template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ V, long ldv, CoefPack a,
                                                      cpx<R>* __restrict__ x, long n)
 {
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ 
         pr[c] = a.v[2 * c];
         pi[c] = a.v[2 * c + 1];
     }
-    kb::combine_body<R, K>(V, ldv, pr, pi, x, n, blockIdx.x, gridDim.x);
+    kb::combine_body<R, K, NT>(V, ldv, pr, pi, x, n, blockIdx.x, gridDim.x);
 }
 
 // out[q] = sum_b part[q*nblk + b] in block order; one block per column q.
@@ -141,14 +141,26 @@ int krylov_blocks(long n)
     return int(std::max(1L, std::min(b, long(kMaxPartials))));
 }
 
+// Streaming basis loads once the basis slots read by a pass exceed the
+// last-level cache (256 MB Infinity Cache on MI355X); smaller bases (coarse
+// multigrid levels, small grids) stay cacheable between the dots and update
+// passes.
+template <typename R>
+bool stream_basis(int k, long n)
+{
+    return double(k) * double(n) * sizeof(cpx<R>) > 256.0 * (1 << 20);
+}
+
 template <typename R>
 hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n, double* part, int nblk,
                        hipStream_t s)
 {
     if (k < 1) return hipErrorInvalidValue;
+    const bool nt = stream_basis<R>(k, n);
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        hipLaunchKernelGGL((dots_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part);
+        if (nt) hipLaunchKernelGGL((dots_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part);
+        else hipLaunchKernelGGL((dots_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part);
         return hipGetLastError();
     });
 }
@@ -158,9 +170,13 @@ hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, cons
                          long n, double* part, int nblk, hipStream_t s)
 {
     if (k < 1) return hipErrorInvalidValue;
+    const bool nt = stream_basis<R>(k, n);
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        hipLaunchKernelGGL((update_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, w, n, part);
+        if (nt)
+            hipLaunchKernelGGL((update_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, w, n, part);
+        else
+            hipLaunchKernelGGL((update_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, w, n, part);
         return hipGetLastError();
     });
 }
@@ -170,9 +186,11 @@ hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, c
                           hipStream_t s)
 {
     if (k <= 0) return hipSuccess;
+    const bool nt = stream_basis<R>(k, n);
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        hipLaunchKernelGGL((combine_kernel<R, K>), dim3(nblk), dim3(KB), 0, s, V, ldv, a, x, n);
+        if (nt) hipLaunchKernelGGL((combine_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, a, x, n);
+        else hipLaunchKernelGGL((combine_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, a, x, n);
         return hipGetLastError();
     });
 }
