@@ -150,6 +150,16 @@ struct helm3d_ctx {
     double* small = nullptr;  // device: 2*kMaxBasis dots + norm
     size_t small_cap = 0;
     double* small_host = nullptr;  // pinned mirror
+    // Arnoldi steps enqueued ahead of the host: per step k a row of
+    // kGsRow doubles (dots d_0..d_{k-1}, then ||w||^2), the squared norms of
+    // the basis slots (scales of the update and of the next matvec), and a
+    // pinned two-row ring the rows are copied into (one event per row)
+    double* gs = nullptr;  // device: (kMaxBasis + 1) rows
+    size_t gs_cap = 0;
+    double* nrm = nullptr;  // device: kMaxBasis + 1 values
+    size_t nrm_cap = 0;
+    double* ring_host = nullptr;  // pinned: 2 rows
+    hipEvent_t ring_ev[2] = {nullptr, nullptr};
 
     Vecs<double> V64, Z64, T64, BC64;
     Vecs<float> V32, Z32, T32a, T32b, BC32;
@@ -175,10 +185,13 @@ struct helm3d_ctx {
     {
         (void)hipSetDevice(device);
         void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, BC64.p,
-                        V32.p, Z32.p, T32a.p, T32b.p, BC32.p, bsc};
+                        V32.p, Z32.p, T32a.p, T32b.p, BC32.p, bsc, gs, nrm};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (small_host) (void)hipHostFree(small_host);
+        if (ring_host) (void)hipHostFree(ring_host);
+        for (hipEvent_t e : ring_ev)
+            if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (inner_graph) (void)hipGraphExecDestroy(inner_graph);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -223,6 +236,8 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
 
 int model_variant(const helm3d_ctx* c, int mode_adj);
 
+constexpr int kGsRow = 2 * kMaxBasis + 2;  // doubles per Arnoldi-step row
+
 // Context work buffers: reduction partials, reduced values (+ pinned host
 // mirror) and the stencil's write-only sink.
 inline int ensure_work(helm3d_ctx* c)
@@ -233,6 +248,12 @@ inline int ensure_work(helm3d_ctx* c)
     CHK(ensure(&c->sink, &c->sink_cap, size_t(kSinkSlots) * sizeof(cpx<double>)));
     if (!c->small_host)
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->small_host), (2 * kMaxBasis + 8) * sizeof(double)));
+    CHK(ensure_t(&c->gs, &c->gs_cap, size_t(kMaxBasis + 1) * kGsRow));
+    CHK(ensure_t(&c->nrm, &c->nrm_cap, size_t(kMaxBasis + 1)));
+    if (!c->ring_host)
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->ring_host), 2 * kGsRow * sizeof(double)));
+    for (hipEvent_t& e : c->ring_ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return HELM3D_OK;
 }
 

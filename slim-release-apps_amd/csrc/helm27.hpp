@@ -18,6 +18,7 @@ struct StencilLaunch {
     cpx<R>* y;           // output: y = alpha * (H x) + beta * b
     const cpx<R>* b;     // optional addend (nullptr: none)
     R alpha, beta;
+    const double* anrm;  // optional (EPI 0/1): alpha is divided by sqrt(*anrm) (device scalar)
     double* norm_part;   // optional: per-block partial sums of |y|^2
     // optional FWI-gradient epilogue: instead of storing y, accumulate
     // gg[phys] += gscale * Re(conj(gu) * y) over the physical box

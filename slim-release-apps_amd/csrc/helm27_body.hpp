@@ -199,6 +199,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         }
     }
     cpx<R> Xlast = zero<R>();  // EPI 3/4: staged x at the own point of the previous plane
+    // output scale: alpha, or alpha / sqrt(*anrm) with the norm held on the device
+    const R alpha = (EPI <= 1 && L.anrm) ? R(double(L.alpha) / sqrt(*L.anrm)) : L.alpha;
 
     // One register set: halo slots of plane q and the own-point operands of
     // plane q-1 (the plane output when q is staged).  Every load is issued
@@ -460,7 +462,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             double* dst = g ? &L.gg[gcol_off + long(qk) * gplane] : reinterpret_cast<double*>(&L.sink[sink_slot]);
             *dst = gv + L.gscale * v;
         } else {
-            cpx<R> r = L.alpha * out;
+            cpx<R> r = alpha * out;
             if constexpr (EPI == 1) r = rfma(r, L.beta, bb);
             cpx<R>* dst = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
             *dst = r;

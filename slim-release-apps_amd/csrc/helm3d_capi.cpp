@@ -105,10 +105,11 @@ int stencil(helm3d_ctx* c, int variant, const OpDesc<R>& op, const cpx<R>* x, cp
     return HELM3D_OK;
 }
 
-// Reduce `nq` partial columns (part[q*nblk + b]) into c->small[off..off+nq).
-int reduce_to_small(helm3d_ctx* c, int nblk, int nq, int off)
+// Reduce `nq` partial columns (part[q*nblk + b]) into c->small[off..off+nq)
+// (and into copy[0..nq) when given).
+int reduce_to_small(helm3d_ctx* c, int nblk, int nq, int off, double* copy = nullptr)
 {
-    TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, nq, c->small + off, c->stream));
+    TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, nq, c->small + off, c->stream, copy));
     return HELM3D_OK;
 }
 int fetch_small(helm3d_ctx* c, int count)
@@ -126,6 +127,15 @@ namespace helm3d {
 // Restarted GMRES of FGMRES.m:33-128 on precision R.  The basis is kept
 // unnormalised (slot c raw, scale sig[c]); P makes it the flexible variant of
 // FGMRES.m:96-98,118 with Z stored true-scaled.
+//
+// Without P the Arnoldi steps run one ahead of the host: the squared norms of
+// the basis slots stay on the device (c->nrm), where the next matvec takes its
+// input scale and the update pass its coefficients, so step k+1 is enqueued
+// before the host reads step k's dot products (copied into a pinned two-row
+// ring).  The host then builds the Hessenberg column and the quasi-residual
+// of step k while the device runs step k+1; a step enqueued past the stopping
+// point is discarded (its slot is never combined).  The arithmetic is that of
+// the one-step-at-a-time loop.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
                  long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst)
@@ -148,15 +158,15 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     const int nblk = krylov_blocks(N);
     auto slot = [&](int s) { return V + size_t(s) * ld; };
 
-    // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62)
+    // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62); ||r0||^2 also into nrm[0]
     if (use_x0) {
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
-        CHK(reduce_to_small(c, c->grid.blocks, 1, 0));
+        CHK(reduce_to_small(c, c->grid.blocks, 1, 0, c->nrm));
     } else {
         HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, slot(0), N, false, c->stream)));
         HIPCHK((launch_axpby<R, R>(0.0, 0.0, static_cast<const cpx<R>*>(nullptr), 0.0, 0.0, x, N, false, c->stream)));
         HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
-        CHK(reduce_to_small(c, nblk, 1, 0));
+        CHK(reduce_to_small(c, nblk, 1, 0, c->nrm));
     }
     CHK(fetch_small(c, 1));
     const double normr0 = std::sqrt(c->small_host[0]);
@@ -179,13 +189,57 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         std::fill(H.begin(), H.end(), C(0));
         sig[0] = 1.0 / beta;
         int k = 1;
-        while (k <= m && hlast > tol) {
-            // w = A v_k (or A z_k) into slot k
-            if (P) {
+        if (!P) {
+            // enqueue Arnoldi step j: slot(j) = A slot(j-1)/||slot(j-1)||, dots into
+            // row j, update with device scales, ||slot(j)||^2 into row j and nrm[j]
+            auto enqueue = [&](int j) -> int {
+                double* row = c->gs + size_t(j) * kGsRow;
+                StencilLaunch<R> L{};
+                L.op = op;
+                L.x = slot(j - 1);
+                L.y = slot(j);
+                L.alpha = R(1);
+                L.beta = R(0);
+                L.anrm = c->nrm + (j - 1);
+                L.sink = static_cast<cpx<R>*>(c->sink);
+                TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, c->stream));
+                TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, c->stream));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, c->stream));
+                TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk,
+                                                           c->stream, c->nrm));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, row + 2 * j, c->stream, c->nrm + j));
+                HIPCHK(hipMemcpyAsync(c->ring_host + (j & 1) * kGsRow, row, (2 * j + 1) * sizeof(double),
+                                      hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipEventRecord(c->ring_ev[j & 1], c->stream));
+                return HELM3D_OK;
+            };
+            // may a step follow once `done` steps have been taken (iteration cap)
+            auto may_continue = [&](long done) { return iter_cap <= 0 || done < iter_cap; };
+            CHK(enqueue(1));
+            while (k <= m && hlast > tol) {
+                if (k + 1 <= m && may_continue(inner + 1)) CHK(enqueue(k + 1));
+                HIPCHK(hipEventSynchronize(c->ring_ev[k & 1]));
+                const double* h = c->ring_host + (k & 1) * kGsRow;
+                for (int q = 0; q < k; ++q) H[q + (k - 1) * (m + 1)] = sig[q] * C(h[2 * q], h[2 * q + 1]);
+                const double hn = std::sqrt(h[2 * k]);
+                H[k + (k - 1) * (m + 1)] = hn;
+                sig[k] = 1.0 / hn;
+                hlast = hess_lsq(k, m + 1, H, beta, y) / normr0;
+                if (hst) hst->push_back(hlast);
+                ++k;
+                ++it;
+                ++inner;
+                if (!may_continue(inner)) {
+                    capped = true;
+                    break;
+                }
+            }
+        }
+        while (P && k <= m && hlast > tol) {
+            // w = A z_k into slot k
+            {
                 CHK(P->apply(slot(k - 1), sig[k - 1], Z + size_t(k - 1) * ld));
                 CHK(stencil<R>(c, variant, op, Z + size_t(k - 1) * ld, slot(k), R(1), nullptr, R(0), nullptr));
-            } else {
-                CHK(stencil<R>(c, variant, op, slot(k - 1), slot(k), R(sig[k - 1]), nullptr, R(0), nullptr));
             }
             // classical Gram-Schmidt, FGMRES.m:100-103
             TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
@@ -228,7 +282,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         }
         // true residual r = b - A x into slot 0, FGMRES.m:120-124
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
-        CHK(reduce_to_small(c, c->grid.blocks, 1, 0));
+        CHK(reduce_to_small(c, c->grid.blocks, 1, 0, c->nrm));
         CHK(fetch_small(c, 1));
         const double rn = std::sqrt(c->small_host[0]);
         res->relres = rn / normr0;

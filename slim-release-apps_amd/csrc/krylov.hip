@@ -37,16 +37,21 @@ __global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, 
 
 // w <- w - sum_{c<K} a_c v_c with a_c = scl_c * d_c (d: reduced dots in device
 // memory, so the host need not see them before this pass); partial ||w||^2.
+// nrm != nullptr: scl_c = (1 / sqrt(nrm[c]))^2 from device memory, the square
+// of the slot scale the host applies to the Hessenberg entries.
 template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                     const double* __restrict__ d, CoefPack scl,
-                                                    cpx<R>* __restrict__ w, long n, double* __restrict__ part)
+                                                    const double* __restrict__ nrm, cpx<R>* __restrict__ w,
+                                                    long n, double* __restrict__ part)
 {
     double ar[K], ai[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
-        ar[c] = scl.v[c] * d[2 * c];
-        ai[c] = scl.v[c] * d[2 * c + 1];
+        const double inv = nrm ? 1.0 / sqrt(nrm[c]) : 0.0;
+        const double sc = nrm ? inv * inv : scl.v[c];
+        ar[c] = sc * d[2 * c];
+        ai[c] = sc * d[2 * c + 1];
     }
     kb::update_body<R, K, NT>(V, ldv, ar, ai, w, n, part, blockIdx.x, gridDim.x);
 }
@@ -65,12 +70,16 @@ __global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ 
     kb::combine_body<R, K, NT>(V, ldv, pr, pi, x, n, blockIdx.x, gridDim.x);
 }
 
-// out[q] = sum_b part[q*nblk + b] in block order; one block per column q.
+// out[q] = sum_b part[q*nblk + b] in block order; one block per column q
+// (out2, when given, receives the same values).
 __global__ __launch_bounds__(KB) void reduce_kernel(const double* __restrict__ part, int nblk,
-                                                    double* __restrict__ out)
+                                                    double* __restrict__ out, double* __restrict__ out2)
 {
     const double s = kb::reduce_body(part, nblk, blockIdx.x);
-    if (threadIdx.x == 0) out[blockIdx.x] = s;
+    if (threadIdx.x == 0) {
+        out[blockIdx.x] = s;
+        if (out2) out2[blockIdx.x] = s;
+    }
 }
 
 // y = alpha x + beta y (complex alpha, beta); use_y == 0 ignores y's contents.
@@ -167,16 +176,18 @@ hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n
 
 template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
-                         long n, double* part, int nblk, hipStream_t s)
+                         long n, double* part, int nblk, hipStream_t s, const double* nrm)
 {
     if (k < 1) return hipErrorInvalidValue;
     const bool nt = stream_basis<R>(k, n);
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
         if (nt)
-            hipLaunchKernelGGL((update_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, w, n, part);
+            hipLaunchKernelGGL((update_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, nrm, w, n,
+                               part);
         else
-            hipLaunchKernelGGL((update_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, w, n, part);
+            hipLaunchKernelGGL((update_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, nrm, w, n,
+                               part);
         return hipGetLastError();
     });
 }
@@ -195,9 +206,9 @@ hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, c
     });
 }
 
-hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s)
+hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s, double* out2)
 {
-    hipLaunchKernelGGL(reduce_kernel, dim3(nq), dim3(KB), 0, s, part, nblk, out);
+    hipLaunchKernelGGL(reduce_kernel, dim3(nq), dim3(KB), 0, s, part, nblk, out, out2);
     return hipGetLastError();
 }
 
@@ -237,7 +248,7 @@ hipError_t launch_norm2(const cpx<R>* x, long n, double* part, int nblk, hipStre
     template hipError_t launch_dots<R>(const cpx<R>*, long, int, const cpx<R>*, long, double*, int,        \
                                        hipStream_t);                                                        \
     template hipError_t launch_update<R>(const cpx<R>*, long, int, const double*, const CoefPack&, cpx<R>*,  \
-                                         long, double*, int, hipStream_t);                                  \
+                                         long, double*, int, hipStream_t, const double*);                   \
     template hipError_t launch_combine<R>(const cpx<R>*, long, int, const CoefPack&, cpx<R>*, long, int,    \
                                           hipStream_t);                                                     \
     template hipError_t launch_norm2<R>(const cpx<R>*, long, double*, int, hipStream_t);

@@ -18,10 +18,12 @@ int krylov_blocks(long n);
 template <typename R>
 hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n, double* part, int nblk,
                        hipStream_t s);
-// a_c = scl.v[c] * (d[2c] + i d[2c+1]); d is device memory.
+// a_c = scl.v[c] * (d[2c] + i d[2c+1]); d is device memory.  With nrm (device
+// memory, ||v_c||^2 per slot) the scale is (1 / sqrt(nrm[c]))^2 instead of
+// scl.v[c], so the pass needs nothing from the host.
 template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
-                         long n, double* part, int nblk, hipStream_t s);
+                         long n, double* part, int nblk, hipStream_t s, const double* nrm = nullptr);
 template <typename R>
 hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, cpx<R>* x, long n, int nblk,
                           hipStream_t s);
@@ -29,7 +31,8 @@ hipError_t launch_interleave(const double* re, const double* im, cpx<double>* z,
 hipError_t launch_deinterleave(const cpx<double>* z, double* re, double* im, long n, hipStream_t s);
 hipError_t launch_cast_real(const double* a, float* b, long n, hipStream_t s);
 // out[q] = sum_b part[q*nblk + b] for q < nq (block order, deterministic)
-hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s);
+// out[q] = sum of partial column q; out2 (optional) receives a copy.
+hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s, double* out2 = nullptr);
 template <typename RX, typename RY>
 hipError_t launch_axpby(double ar, double ai, const cpx<RX>* x, double br, double bi, cpx<RY>* y, long n,
                         bool use_y, hipStream_t s);
