@@ -193,7 +193,7 @@ def uneven_leg(plain_rhs_per_s):
     from helm3d.mg import MGPool
     mg_t, mg_conv, mg_rel = 0.0, True, 0.0
     for k, c in enumerate(subs):
-        pool = MGPool(vec(c.v), c.n_phys, c.h, c.npml, c.freq, nlevels=4, device=dev, workers=4)
+        pool = MGPool(vec(c.v), c.n_phys, c.h, c.npml, c.freq, nlevels=4, k=MG_K_TUNED, device=dev, workers=4)
         mine = [bs[i] for i in range(len(bs)) if owner[i] == k]
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -219,26 +219,35 @@ def uneven_leg(plain_rhs_per_s):
             "max_relres": max(i.relres for i in infos), "subproblems": per_sub, "launch": tuned,
             "dtype": "c64 inner / f64 outer",
             "multigrid": {"rhs_per_s": len(bs) / mg_t, "seconds": mg_t, "converged": mg_conv, "max_relres": mg_rel,
-                          "solver": "FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, 4 streams per subproblem"}}
+                          "solver": f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, k = {MG_K_TUNED}, 4 streams "
+                                    "per subproblem"}}
 
 
-def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128"), (3, "mixed"), (4, "c128"), (4, "mixed"))):
+MG_K_DEFAULT = (3, 5, 3, 5)  # ML_GMRES.m:3-6
+MG_K_TUNED = (1, 5, 3, 5)    # one smoothing cycle: fastest in a sweep of k on the 240x240x140 solve
+
+
+def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128", MG_K_DEFAULT), (3, "mixed", MG_K_DEFAULT),
+                                               (4, "c128", MG_K_DEFAULT), (4, "mixed", MG_K_DEFAULT),
+                                               (4, "mixed", MG_K_TUNED))):
     """The benchmark solve (240x240x140, 6 Hz, one impulse, 1e-6 FP64
     residual) with FGMRES(20) preconditioned by the ML_GMRES V-cycle: 3 levels
-    (ML_GMRES.m) and 4 levels, the V-cycle in complex128 and in complex64."""
+    (ML_GMRES.m) and 4 levels, the V-cycle in complex128 and in complex64,
+    ML_GMRES's cycle sizes k and a tuned k."""
     import torch
     from helm3d.mg import MLGMRES
     from helm3d.models import config_c2, vec
     cfg = config_c2()
     dev = torch.device("cuda:0")
     runs = []
-    for levels in sorted({lv for lv, _ in configs}):
+    for levels in sorted({cf[0] for cf in configs}):
         M = MLGMRES(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=levels, device=dev)
         nx, ny, _ = M.grids[0]["nt"]
         p, s = cfg.npml, cfg.sources[0]
         b = torch.zeros(M.H.N, dtype=torch.complex128, device=dev)
         b[(s[0] + p) + (s[1] + p) * nx + (s[2] + p) * nx * ny] = 1.0
-        for prec in [p for lv, p in configs if lv == levels]:
+        for _, prec, k in [cf for cf in configs if cf[0] == levels]:
+            M.k = tuple(k)
             M.solve(b, tol=1e-6, precision=prec)  # warm-up (workspaces)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -247,7 +256,7 @@ def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128"), (3, "mixed"), (4, "c
             torch.cuda.synchronize()
             dt = (time.perf_counter() - t0) / solves
             relres = float((b - M.H.apply(x)).norm() / b.norm())
-            runs.append({"levels": levels, "grids": [list(g["nt"]) for g in M.grids],
+            runs.append({"levels": levels, "grids": [list(g["nt"]) for g in M.grids], "k": list(k),
                          "vcycle": "complex128" if prec == "c128" else "complex64", "rhs_per_s": 1.0 / dt,
                          "seconds_per_solve": dt,
                          "vs_plain": (1.0 / dt) / plain_rhs_per_s if plain_rhs_per_s else None,
@@ -257,8 +266,8 @@ def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128"), (3, "mixed"), (4, "c
         torch.cuda.empty_cache()
     best = max(runs, key=lambda r: r["rhs_per_s"])
     return {"workload": "the 240x240x140 benchmark solve (one impulse, 6 Hz, 1e-6 FP64 residual) with FGMRES(20) "
-                        "preconditioned by the ML_GMRES V-cycle, k = (3, 5, 3, 5)",
-            "rhs_per_s": best["rhs_per_s"], "best": {k: best[k] for k in ("levels", "vcycle")}, "runs": runs,
+                        "preconditioned by the ML_GMRES V-cycle",
+            "rhs_per_s": best["rhs_per_s"], "best": {q: best[q] for q in ("levels", "vcycle", "k")}, "runs": runs,
             "dtype": "f64 outer / c128 or c64 V-cycle"}
 
 
@@ -272,7 +281,8 @@ def mg_sources_leg(plain_rhs_per_s, workers=4):
     from helm3d.models import config_c3, vec
     cfg = config_c3()
     dev = torch.device("cuda:0")
-    pool = MGPool(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=4, device=dev, workers=workers)
+    pool = MGPool(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=4, k=MG_K_TUNED, device=dev,
+                  workers=workers)
     nx, ny, _ = pool.grids[0]["nt"]
     p = cfg.npml
     bs = []
@@ -293,7 +303,8 @@ def mg_sources_leg(plain_rhs_per_s, workers=4):
     pool.close()
     return {"workload": "8 point sources (z=5, y=100, x=round(linspace(20,179,8))) on the 240x240x140 benchmark "
                         "model, FGMRES(20) in FP64 + 4-level complex64 ML_GMRES V-cycle (inner FGMRES smoothers "
-                        f"and coarse solves in complex64), {workers} hierarchies on {workers} streams, 1e-6 FP64 residual",
+                        f"and coarse solves in complex64, k = {MG_K_TUNED}), {workers} hierarchies on {workers} "
+                        "streams, 1e-6 FP64 residual",
             "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s if plain_rhs_per_s else None,
             "outer_iters": [i.iters for i in infos], "final_relres": rel, "err_vs_complex128": errs,
             "dtype": "c64 V-cycle / f64 outer"}
@@ -310,8 +321,10 @@ def fwi_leg(solver="mg"):
     model, m_true, m0 = fwi_c5()
     dev = torch.device("cuda:0")
     if solver == "mg":
-        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=4)
-        how = "FGMRES(20) + 4-level complex64 ML_GMRES V-cycle, FP64 outer, 4 solves at a time on 4 streams"
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=4,
+                       mg_k=MG_K_TUNED)
+        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, 4 solves at a "
+               "time on 4 streams")
     else:
         opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20)
         how = "batched mixed-precision solves (16 sources per batch)"
@@ -411,7 +424,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
     ap.add_argument("--mg-config", default=None,
-                    help="--mode mg only: one 'levels,precision' pair (e.g. 4,mixed) instead of all four")
+                    help="--mode mg only: one 'levels,precision[,k1,k2,k3,k4]' set (e.g. 4,mixed or "
+                         "4,mixed,1,5,3,5) instead of the default list")
     ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg", "mg-sources"),
                     default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
@@ -443,8 +457,9 @@ def main():
             res = mg_sources_leg(plain_rhs_per_s)
         elif args.mode == "mg":
             if args.mg_config:
-                lv, pr = args.mg_config.split(",")
-                res = mg_leg(plain_rhs_per_s, solves=args.steps, configs=((int(lv), pr),))
+                f = args.mg_config.split(",")
+                k = tuple(int(a) for a in f[2:6]) if len(f) >= 6 else MG_K_DEFAULT
+                res = mg_leg(plain_rhs_per_s, solves=args.steps, configs=((int(f[0]), f[1], k),))
             else:
                 res = mg_leg(plain_rhs_per_s)
         else:
