@@ -28,16 +28,16 @@ __device__ inline const VecItem<R>* item_of(const VecItem<R>* items, const int* 
 template <typename R>
 __device__ inline cpx<R>* slot(const VecItem<R>* I, int s) { return I->ws + long(s) * I->ld; }
 
-template <typename R, int K>
+template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void bdots_kernel(const VecItem<R>* __restrict__ items,
                                                    const int* __restrict__ active, int vslot, int wslot)
 {
     const VecItem<R>* I = item_of(items, active);
     if (!I) return;
-    kb::dots_body<R, K>(slot(I, vslot), I->ld, slot(I, wslot), I->n, I->part, blockIdx.x, I->nblk);
+    kb::dots_body<R, K, NT>(slot(I, vslot), I->ld, slot(I, wslot), I->n, I->part, blockIdx.x, I->nblk);
 }
 
-template <typename R, int K>
+template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void bupdate_kernel(const VecItem<R>* __restrict__ items,
                                                      const int* __restrict__ active, int vslot, int wslot, int nrm)
 {
@@ -52,10 +52,10 @@ __global__ __launch_bounds__(KB) void bupdate_kernel(const VecItem<R>* __restric
         ar[c] = inv * d[2 * c];
         ai[c] = inv * d[2 * c + 1];
     }
-    kb::update_body<R, K>(slot(I, vslot), I->ld, ar, ai, slot(I, wslot), I->n, I->part, blockIdx.x, I->nblk);
+    kb::update_body<R, K, NT>(slot(I, vslot), I->ld, ar, ai, slot(I, wslot), I->n, I->part, blockIdx.x, I->nblk);
 }
 
-template <int K>
+template <int K, bool NT>
 __global__ __launch_bounds__(KB) void bcombine_kernel(const VecItem<double>* __restrict__ items,
                                                       const int* __restrict__ active, int vslot,
                                                       const double* __restrict__ coef)
@@ -69,7 +69,7 @@ __global__ __launch_bounds__(KB) void bcombine_kernel(const VecItem<double>* __r
         pr[c] = a[2 * c];
         pi[c] = a[2 * c + 1];
     }
-    kb::combine_body<double, K>(slot(I, vslot), I->ld, pr, pi, I->x, I->n, blockIdx.x, I->nblk);
+    kb::combine_body<double, K, NT>(slot(I, vslot), I->ld, pr, pi, I->x, I->n, blockIdx.x, I->nblk);
 }
 
 // grid (nq, nitems)
@@ -200,38 +200,50 @@ hipError_t dispatch_k(int k, Fn&& fn)
 
 template <typename R>
 hipError_t launch_bdots(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int k, int vslot,
-                        int wslot, hipStream_t s)
+                        int wslot, hipStream_t s, bool nt)
 {
     if (k < 1) return hipErrorInvalidValue;
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        hipLaunchKernelGGL((bdots_kernel<R, K>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, vslot,
-                           wslot);
+        if (nt)
+            hipLaunchKernelGGL((bdots_kernel<R, K, true>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active,
+                               vslot, wslot);
+        else
+            hipLaunchKernelGGL((bdots_kernel<R, K, false>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active,
+                               vslot, wslot);
         return hipGetLastError();
     });
 }
 
 template <typename R>
 hipError_t launch_bupdate_nrm(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int k,
-                              int vslot, int wslot, int nrm, hipStream_t s)
+                              int vslot, int wslot, int nrm, hipStream_t s, bool nt)
 {
     if (k < 1) return hipErrorInvalidValue;
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        hipLaunchKernelGGL((bupdate_kernel<R, K>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, vslot,
-                           wslot, nrm);
+        if (nt)
+            hipLaunchKernelGGL((bupdate_kernel<R, K, true>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active,
+                               vslot, wslot, nrm);
+        else
+            hipLaunchKernelGGL((bupdate_kernel<R, K, false>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active,
+                               vslot, wslot, nrm);
         return hipGetLastError();
     });
 }
 
 hipError_t launch_bcombine(const VecItem<double>* items, const int* active, int nitems, int nblk_max, int k,
-                           int vslot, const double* coef, hipStream_t s)
+                           int vslot, const double* coef, hipStream_t s, bool nt)
 {
     if (k <= 0) return hipSuccess;
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        hipLaunchKernelGGL((bcombine_kernel<K>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, vslot,
-                           coef);
+        if (nt)
+            hipLaunchKernelGGL((bcombine_kernel<K, true>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active,
+                               vslot, coef);
+        else
+            hipLaunchKernelGGL((bcombine_kernel<K, false>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active,
+                               vslot, coef);
         return hipGetLastError();
     });
 }
@@ -293,9 +305,10 @@ hipError_t launch_bbicg_xr(const VecItem<R>* items, const int* active, int nitem
 }
 
 #define HELM3D_BATCH_INST(R)                                                                                     \
-    template hipError_t launch_bdots<R>(const VecItem<R>*, const int*, int, int, int, int, int, hipStream_t);    \
+    template hipError_t launch_bdots<R>(const VecItem<R>*, const int*, int, int, int, int, int, hipStream_t,     \
+                                        bool);                                                                   \
     template hipError_t launch_bupdate_nrm<R>(const VecItem<R>*, const int*, int, int, int, int, int, int,       \
-                                              hipStream_t);                                                      \
+                                              hipStream_t, bool);                                                \
     template hipError_t launch_breduce<R>(const VecItem<R>*, const int*, int, int, int, int, bool, hipStream_t); \
     template hipError_t launch_bnorm2<R>(const VecItem<R>*, const int*, int, int, int, hipStream_t);             \
     template hipError_t launch_bbicg_init<R>(const VecItem<R>*, const int*, int, int, int, int, int, int, int,   \

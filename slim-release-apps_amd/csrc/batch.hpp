@@ -32,16 +32,18 @@ enum BatchTarget : int { BT_SMALL = 0, BT_SC = 1 };
 
 template <typename R>
 hipError_t launch_bdots(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int k, int vslot,
-                        int wslot, hipStream_t s);
+                        int wslot, hipStream_t s, bool nt = false);
 // w -= sum_c a_c u_c with a_c = d_c / small[nrm + c] (d_c = u_c^H w in
 // small[2c..2c+1], ||u_c||^2 in small[nrm + c]: the projection on the raw
 // basis vectors); partial ||w||^2
 template <typename R>
 hipError_t launch_bupdate_nrm(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int k,
-                              int vslot, int wslot, int nrm, hipStream_t s);
+                              int vslot, int wslot, int nrm, hipStream_t s, bool nt = false);
 // x += sum_c a_c v_c, a_c = coef[it*2*kMaxBasis + 2c .. +1]  (R = double: x is the item's solution)
+// (nt on these three: streaming basis loads, for bases larger than the
+// last-level cache)
 hipError_t launch_bcombine(const VecItem<double>* items, const int* active, int nitems, int nblk_max, int k,
-                           int vslot, const double* coef, hipStream_t s);
+                           int vslot, const double* coef, hipStream_t s, bool nt = false);
 // small/sc[off + q] = sum_b part[q*nb + b]; nb = sblk for partials written by
 // a stencil launch, else nblk
 template <typename R>

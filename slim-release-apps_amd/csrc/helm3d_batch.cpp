@@ -122,6 +122,7 @@ struct Layout {
     std::vector<int> nblk, sblk, first;
     std::vector<StencilGrid> grid;
     int nblk_max = 1, sblk_total = 0;
+    long total_points = 0;  // sum of the items' grid sizes
     // device descriptor regions (byte offsets into b->desc)
     size_t o_i64 = 0, o_i32 = 0, o_st64 = 0, o_res64 = 0, o_k1 = 0, o_k2 = 0, o_k1f = 0, o_first = 0, o_active = 0;
     size_t total = 0;
@@ -235,6 +236,7 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
     for (int i = 0; i < n; ++i) {
         helm3d_ctx* c = ctxs[i];
         Lo.N.push_back(c->N);
+        Lo.total_points += c->N;
         Lo.ld.push_back(slot_ld(c->N));
         Lo.off64.push_back(tot64);
         Lo.off32.push_back(tot32);
@@ -501,9 +503,11 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
             HIPCHK(launch_stencil_batch<double>(Lo.variant, 0, at<StencilLaunch<double>>(B, Lo.o_st64) + size_t(k) * n,
                                                 d_first, n, Lo.sblk_total, d_active, Lo.ty, Lo.pf, st));
             // classical Gram-Schmidt (FGMRES.m:100-103) on the unnormalised basis
-            HIPCHK(launch_bdots<double>(d_i64, d_active, n, nbm, k, 0, k, st));
+            // streaming basis loads once the slots read per pass exceed the last-level cache
+            const bool nt = double(k) * double(Lo.total_points) * sizeof(cpx<double>) > 256.0 * (1 << 20);
+            HIPCHK(launch_bdots<double>(d_i64, d_active, n, nbm, k, 0, k, st, nt));
             HIPCHK(launch_breduce<double>(d_i64, d_active, n, 2 * k, kDots, BT_SMALL, false, st));
-            HIPCHK(launch_bupdate_nrm<double>(d_i64, d_active, n, nbm, k, 0, k, kNrm, st));
+            HIPCHK(launch_bupdate_nrm<double>(d_i64, d_active, n, nbm, k, 0, k, kNrm, st, nt));
             HIPCHK(launch_breduce<double>(d_i64, d_active, n, 1, kNrm + k, BT_SMALL, false, st));
             CHK(fetch());
             double qmax = 0.0;
@@ -549,7 +553,9 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
         }
         HIPCHK(hipMemcpyAsync(B->coef, B->coef_host, size_t(n) * 2 * kMaxBasis * sizeof(double),
                               hipMemcpyHostToDevice, st));
-        if (kmax > 0) HIPCHK(launch_bcombine(d_i64, d_active, n, nbm, kmax, Lo.mixed ? m + 1 : 0, B->coef, st));
+        if (kmax > 0)
+            HIPCHK(launch_bcombine(d_i64, d_active, n, nbm, kmax, Lo.mixed ? m + 1 : 0, B->coef, st,
+                                   double(kmax) * double(Lo.total_points) * sizeof(cpx<double>) > 256.0 * (1 << 20)));
         // true residual r = b - A x into slot 0 and its norm (FGMRES.m:120-124)
         CHK(set_active([&](int i) { return !S[i].done; }));
         HIPCHK(launch_stencil_batch<double>(Lo.variant, 1, at<StencilLaunch<double>>(B, Lo.o_res64), d_first, n,
