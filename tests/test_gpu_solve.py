@@ -148,3 +148,30 @@ def test_mixed_fixed_inner_graph_replays_match_eager():
     xa, ia = H.solve_mixed(bt, mode=2, tol=1e-6, inner_tol=0.0, inner_maxit=10)
     ra = b - O.hmvp("adj", wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), xa.cpu().numpy())
     assert ia.converged and np.linalg.norm(ra) / np.linalg.norm(b) < 1e-6
+
+
+@pytest.mark.parametrize("m,maxit", [(20, 2), (7, 3), (1, 5)])
+def test_solve_stops_at_maxit_like_oracle(m, maxit):
+    """FGMRES.m's loop guard floor(it/m) < maxit (it counted from 1) with the
+    Arnoldi steps enqueued ahead of the host: the device stops after the
+    oracle's number of steps with its iterate and quasi-residual history (a
+    step enqueued past the stopping point is discarded)."""
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small
+    cfg = config_small()
+    wn, b = _setup(cfg)
+    npml, h = cfg.npml6, (cfg.h,) * 3
+    print(f"maxit stop m={m} maxit={maxit}", flush=True)
+    xo, io = O.fgmres("forw", wn, h, cfg.nt, np.array(npml), b, tol=1e-12, maxinnerit=m, maxit=maxit)
+    H = Helm3d(cfg.nt, h, npml, torch.from_numpy(wn).cuda())
+    x, info = H.solve(torch.from_numpy(b).cuda(), tol=1e-12, maxinnerit=m, maxit=maxit, return_hst=True)
+    print("gpu", info.iters, info.cycles, info.relres, "oracle", io["iters"], io["relres"], flush=True)
+    assert not info.converged
+    assert info.iters == io["iters"] and info.cycles <= maxit
+    assert abs(info.relres - io["relres"]) <= 1e-8 * io["relres"]
+    k = min(len(info.hst), len(io["hst"]))
+    assert k >= info.iters
+    assert np.allclose(info.hst[:k], io["hst"][:k], rtol=1e-8, atol=0)
+    x = x.cpu().numpy()
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-8
