@@ -390,7 +390,10 @@ def matvec_mode(args, cfg, wn):
     from helm3d import Helm3d
     dev = torch.device("cuda:0")
     H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
-    if not args.no_tune:
+    if args.launch:
+        kc, ty, pf = (int(a) for a in args.launch.split(","))
+        H.set_launch(kc, ty, pf)
+    elif not args.no_tune:
         H.autotune(mode=1, precision="c128")
     x = torch.from_numpy(np.random.default_rng(0).standard_normal(H.N) + 0j).to(dev)
     y = torch.empty_like(x)
@@ -423,6 +426,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched leg")
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
+    ap.add_argument("--launch", default=None,
+                    help="--mode matvec only: stencil launch shape 'kchunk,tile_rows,prefetch' (no autotuning)")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
     ap.add_argument("--mg-config", default=None,
                     help="--mode mg only: one 'levels,precision[,k1,k2,k3,k4]' set (e.g. 4,mixed or "
