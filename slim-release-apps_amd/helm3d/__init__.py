@@ -7,5 +7,7 @@ host-side mirror of the reference's MATLAB interface for this path.
 from ._lib import Helm3dError, lib, EXPORTED  # noqa: F401
 from .core import Helm3d, SolveInfo, helm3d_kaczswp_host, helm3d_mvp_host, npml_flat  # noqa: F401
 from .batch import BatchSolver  # noqa: F401
+from .mg import MGPool, MLGMRES  # noqa: F401
 
-__all__ = ["Helm3d", "BatchSolver", "SolveInfo", "Helm3dError", "helm3d_mvp_host", "helm3d_kaczswp_host", "npml_flat", "lib", "EXPORTED"]
+__all__ = ["Helm3d", "BatchSolver", "MLGMRES", "MGPool", "SolveInfo", "Helm3dError", "helm3d_mvp_host",
+           "helm3d_kaczswp_host", "npml_flat", "lib", "EXPORTED"]
