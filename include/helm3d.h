@@ -278,6 +278,11 @@ int helm3d_kaczswp(int variant, const double *wnr, const double *wni, const doub
  * info->relres: the true ||b - A x|| / ||b||. */
 int helm3d_solve_cgmn(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
                       const helm3d_solve_opts *opts, helm3d_solve_info *info);
+/* CRMN (CRMN.m:46-108, conjugate residuals) on the same Kaczmarz system
+ * (linearsolve.m:52-63, solver CRMN with the KACZSWP preconditioner): two
+ * inner products and one double sweep per iteration; info as for CGMN. */
+int helm3d_solve_crmn(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
+                      const helm3d_solve_opts *opts, helm3d_solve_info *info);
 
 /* ---- multigrid-preconditioned FGMRES ------------------------------------------ */
 

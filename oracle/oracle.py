@@ -337,6 +337,39 @@ def cgmn(sweep, b, tol=1e-6, maxit=10000):
     return x, k, np.array(hst)
 
 
+def crmn(sweep, b, tol=1e-6, maxit=10000):
+    """CRMN.m:46-108 (conjugate residuals) on the Kaczmarz-preconditioned
+    system of linearsolve.m:52-58 (b' = SWP(0, b), A' y = y - SWP(y, 0)),
+    from x = 0.  Returns (x, iterations, relative residual history)."""
+    zb = np.zeros_like(b)
+    bp = sweep(np.zeros_like(b), b)
+    A = lambda y: y - sweep(y, zb)  # noqa: E731
+    x = np.zeros_like(b)
+    r = bp.copy()
+    Ar = A(r)
+    p, Ap = r.copy(), Ar.copy()
+    nb = np.linalg.norm(bp)
+    rAr = np.vdot(r, Ar)
+    nAp = np.linalg.norm(Ap)
+    hst = [np.linalg.norm(r) / nb]
+    k = 0
+    while True:
+        gamma = rAr / nAp ** 2
+        x = x + gamma * p
+        r = r - gamma * Ap
+        hst.append(np.linalg.norm(r) / nb)
+        k += 1
+        if k >= maxit or hst[-1] <= tol:
+            break
+        Ar = A(r)
+        old, rAr = rAr, np.vdot(r, Ar)
+        beta = rAr / old
+        p = r + beta * p
+        Ap = Ar + beta * Ap
+        nAp = np.linalg.norm(Ap)
+    return x, k, np.array(hst)
+
+
 # ---------------------------------------------------------------------------
 # multigrid grid transfers
 # ---------------------------------------------------------------------------

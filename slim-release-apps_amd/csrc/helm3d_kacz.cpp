@@ -31,6 +31,45 @@ int check_ctx(helm3d_ctx* c)
 
 }  // namespace
 
+namespace {
+
+// ||b - A x|| / ||b|| with the operator, and the solve's info (both
+// Kaczmarz-preconditioned solvers); returns NOTCONV above tol.
+int finish_kacz_solve(helm3d_ctx* c, int mode, const cpx<double>* b, const cpx<double>* x, cpx<double>* t, long k,
+                      double rel, double tol, helm3d_solve_info* info)
+{
+    const long N = c->N;
+    const hipStream_t st = c->stream;
+    const int nblk = krylov_blocks(N);
+    StencilLaunch<double> L{};
+    L.op = c->op64;
+    L.x = x;
+    L.y = t;
+    L.b = b;
+    L.alpha = -1.0;
+    L.beta = 1.0;
+    L.norm_part = c->part;
+    L.sink = static_cast<cpx<double>*>(c->sink);
+    HIPCHK(launch_stencil<double>((mode != 1 ? VAR_ADJ : VAR_FORW) | (c->wnc ? VAR_WNC : 0), L, c->grid, st));
+    HIPCHK(launch_reduce(c->part, c->grid.blocks, 1, c->small, st));
+    HIPCHK(launch_norm2<double>(b, N, c->part, nblk, st));
+    HIPCHK(launch_reduce(c->part, nblk, 1, c->small + 1, st));
+    HIPCHK(hipMemcpyAsync(c->small_host, c->small, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double rr = c->small_host[0], b2 = c->small_host[1];
+    if (info) {
+        info->iters = k;
+        info->inner_iters = 2 * k + 1;  // double sweeps
+        info->cycles = k;
+        info->relres = b2 > 0 ? std::sqrt(rr / b2) : 0.0;
+        info->quasi_relres = rel;
+        info->converged = !(rel > tol);
+    }
+    return !(rel > tol) ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
+}
+
+}  // namespace
+
 extern "C" {
 
 int helm3d_kacz_sweep(helm3d_ctx* c, int mode, double omega, int nsweeps, void* x, const void* b)
@@ -183,32 +222,100 @@ int helm3d_solve_cgmn(helm3d_ctx* c, int mode, const void* bv, void* xv, const h
     } else {
         HIPCHK((launch_axpby<double, double>(0.0, 0.0, none, 0.0, 0.0, x, N, false, st)));
     }
-    // true residual ||b - A x|| / ||b|| with the operator
-    StencilLaunch<double> L{};
-    L.op = c->op64;
-    L.x = x;
-    L.y = t;
-    L.b = b;
-    L.alpha = -1.0;
-    L.beta = 1.0;
-    L.norm_part = c->part;
-    L.sink = static_cast<cpx<double>*>(c->sink);
-    HIPCHK(launch_stencil<double>((mode != 1 ? VAR_ADJ : VAR_FORW) | (c->wnc ? VAR_WNC : 0), L, c->grid, st));
-    HIPCHK(launch_reduce(c->part, c->grid.blocks, 1, c->small, st));
-    HIPCHK(hipMemcpyAsync(c->small_host, c->small, sizeof(double), hipMemcpyDeviceToHost, st));
-    double b2 = 0;
-    HIPCHK(hipStreamSynchronize(st));
-    const double rr = c->small_host[0];
-    CHK(norm2(b, &b2));
-    if (info) {
-        info->iters = k;
-        info->inner_iters = 2 * k + 1;  // double sweeps
-        info->cycles = k;
-        info->relres = b2 > 0 ? std::sqrt(rr / b2) : 0.0;
-        info->quasi_relres = rel;
-        info->converged = !(rel > o.tol);
+    return finish_kacz_solve(c, mode, b, x, t, k, rel, o.tol, info);
+}
+
+int helm3d_solve_crmn(helm3d_ctx* c, int mode, const void* bv, void* xv, const helm3d_solve_opts* opts,
+                      helm3d_solve_info* info)
+{
+    CHK(check_ctx(c));
+    if (!bv || !xv) return fail(HELM3D_ERR_ARG, "null vector");
+    helm3d_solve_opts o;
+    helm3d_solve_opts_default(&o);
+    if (opts) o = *opts;
+    const double omega = 1.5;  // linearsolve.m:53
+    const long N = c->N;
+    const long ld = slot_ld(N);
+    CHK(ensure_vecs(c->T64, N, 6));
+    CHK(ensure_work(c));
+    cpx<double>* r = c->T64.p;
+    cpx<double>* p = r + ld;
+    cpx<double>* Ap = r + 2 * ld;
+    cpx<double>* Ar = r + 3 * ld;
+    cpx<double>* t = r + 4 * ld;
+    cpx<double>* bp = r + 5 * ld;
+    const cpx<double>* b = static_cast<const cpx<double>*>(bv);
+    cpx<double>* x = static_cast<cpx<double>*>(xv);
+    const hipStream_t st = c->stream;
+    const int nblk = krylov_blocks(N);
+    const cpx<double>* none = nullptr;
+    // host copies of a few reduced values
+    auto fetch = [&](int count) -> int {
+        HIPCHK(hipMemcpyAsync(c->small_host, c->small, count * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return HELM3D_OK;
+    };
+    // out = in - SWP(in, 0): A' of linearsolve.m:55
+    auto apply = [&](const cpx<double>* in, cpx<double>* out) -> int {
+        HIPCHK((launch_axpby<double, double>(1.0, 0.0, in, 0.0, 0.0, out, N, false, st)));
+        CHK(sweep(c, mode, omega, out, nullptr, 1));
+        HIPCHK((launch_axpby<double, double>(1.0, 0.0, in, -1.0, 0.0, out, N, true, st)));
+        return HELM3D_OK;
+    };
+    // (u, v) = u^H v and ||v||^2 into small[0..2], small[2]
+    auto dot_norm = [&](const cpx<double>* u, const cpx<double>* v) -> int {
+        HIPCHK(launch_dots<double>(u, ld, 1, v, N, c->part, nblk, st));
+        HIPCHK(launch_reduce(c->part, nblk, 2, c->small, st));
+        HIPCHK(launch_norm2<double>(v, N, c->part, nblk, st));
+        HIPCHK(launch_reduce(c->part, nblk, 1, c->small + 2, st));
+        return HELM3D_OK;
+    };
+    // b' = SWP(0, b); x = 0, r = p = b', Ar = Ap = A' r (CRMN.m:46-57)
+    HIPCHK((launch_axpby<double, double>(0.0, 0.0, none, 0.0, 0.0, bp, N, false, st)));
+    CHK(sweep(c, mode, omega, bp, b, 1));
+    HIPCHK((launch_axpby<double, double>(0.0, 0.0, none, 0.0, 0.0, x, N, false, st)));
+    HIPCHK((launch_axpby<double, double>(1.0, 0.0, bp, 0.0, 0.0, r, N, false, st)));
+    HIPCHK(launch_norm2<double>(bp, N, c->part, nblk, st));
+    HIPCHK(launch_reduce(c->part, nblk, 1, c->small + 3, st));
+    CHK(apply(r, Ar));
+    HIPCHK((launch_axpby<double, double>(1.0, 0.0, r, 0.0, 0.0, p, N, false, st)));
+    HIPCHK((launch_axpby<double, double>(1.0, 0.0, Ar, 0.0, 0.0, Ap, N, false, st)));
+    CHK(dot_norm(r, Ar));  // rAr; small[2] = ||Ar||^2 = ||Ap||^2
+    CHK(fetch(4));
+    const double normb = std::sqrt(c->small_host[3]);
+    std::complex<double> rAr(c->small_host[0], c->small_host[1]);
+    double nAp2 = c->small_host[2];
+    long k = 0;
+    double rel = normb > 0 ? 1.0 : 0.0;
+    if (normb > 0) {
+        while (true) {
+            // CRMN.m:68-88
+            const std::complex<double> gamma = rAr / nAp2;
+            HIPCHK((launch_axpby<double, double>(gamma.real(), gamma.imag(), p, 1.0, 0.0, x, N, true, st)));
+            HIPCHK((launch_axpby<double, double>(-gamma.real(), -gamma.imag(), Ap, 1.0, 0.0, r, N, true, st)));
+            HIPCHK(launch_norm2<double>(r, N, c->part, nblk, st));
+            HIPCHK(launch_reduce(c->part, nblk, 1, c->small, st));
+            CHK(fetch(1));
+            rel = std::sqrt(c->small_host[0]) / normb;
+            ++k;
+            if (k >= o.maxit || !(rel > o.tol)) break;  // stop at hst(end) <= tol
+            // CRMN.m:94-106
+            CHK(apply(r, Ar));
+            HIPCHK(launch_dots<double>(r, ld, 1, Ar, N, c->part, nblk, st));
+            HIPCHK(launch_reduce(c->part, nblk, 2, c->small, st));
+            CHK(fetch(2));
+            const std::complex<double> old = rAr;
+            rAr = std::complex<double>(c->small_host[0], c->small_host[1]);
+            const std::complex<double> beta = rAr / old;
+            HIPCHK((launch_axpby<double, double>(1.0, 0.0, r, beta.real(), beta.imag(), p, N, true, st)));
+            HIPCHK((launch_axpby<double, double>(1.0, 0.0, Ar, beta.real(), beta.imag(), Ap, N, true, st)));
+            HIPCHK(launch_norm2<double>(Ap, N, c->part, nblk, st));
+            HIPCHK(launch_reduce(c->part, nblk, 1, c->small, st));
+            CHK(fetch(1));
+            nAp2 = c->small_host[0];
+        }
     }
-    return !(rel > o.tol) ? HELM3D_OK : fail(HELM3D_ERR_NOTCONV, "maxit reached above tolerance");
+    return finish_kacz_solve(c, mode, b, x, t, k, rel, o.tol, info);
 }
 
 }  // extern "C"

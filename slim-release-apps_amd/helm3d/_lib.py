@@ -125,6 +125,8 @@ _SIGS = [
                                       _dp, _dp, ctypes.c_longlong, _dp, _dp, ctypes.c_longlong, _dp, _dp]),
     ("helm3d_solve_cgmn", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
                                          ctypes.POINTER(SolveInfo)]),
+    ("helm3d_solve_crmn", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
+                                         ctypes.POINTER(SolveInfo)]),
     ("helm3d_mg_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                         ctypes.POINTER(_vp)]),
     ("helm3d_mg_destroy", ctypes.c_int, [_vp]),

@@ -244,21 +244,27 @@ class Helm3d:
         return y
 
     def solve_cgmn(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxit: int = 10000,
-                   raise_on_notconv: bool = False):
-        """CGMN.m on the Kaczmarz system (linearsolve.m:52-58, omega 1.5):
-        iters counts CG iterations, quasi_relres is the preconditioned residual,
-        relres the true one."""
+                   raise_on_notconv: bool = False, method: str = "cgmn"):
+        """CGMN.m (or, with method "crmn", CRMN.m) on the Kaczmarz system
+        (linearsolve.m:52-63, omega 1.5): iters counts CG / CR iterations,
+        quasi_relres is the preconditioned residual, relres the true one."""
+        fn = {"cgmn": lib().helm3d_solve_cgmn, "crmn": lib().helm3d_solve_crmn}[method]
         b = self._vec(b, torch.complex128)
         x = torch.zeros_like(b)
         o = self._opts(tol, 20, maxit, False)
         info = _lib.SolveInfo()
         with torch.cuda.device(self.device):
             self._bind()
-            code = lib().helm3d_solve_cgmn(self._ctx, int(mode), ctypes.c_void_p(_dptr(b)),
-                                           ctypes.c_void_p(_dptr(x)), ctypes.byref(o), ctypes.byref(info))
+            code = fn(self._ctx, int(mode), ctypes.c_void_p(_dptr(b)), ctypes.c_void_p(_dptr(x)), ctypes.byref(o),
+                      ctypes.byref(info))
         check(code, allow=() if raise_on_notconv else (_lib.HELM3D_ERR_NOTCONV,))
         return x, SolveInfo(int(info.iters), int(info.cycles), float(info.relres), float(info.quasi_relres),
                             bool(info.converged), inner_iters=int(info.inner_iters))
+
+    def solve_crmn(self, b: torch.Tensor, mode: int = 1, tol: float = 1e-6, maxit: int = 10000,
+                   raise_on_notconv: bool = False):
+        """CRMN.m on the Kaczmarz system (linearsolve.m:52-63, omega 1.5)."""
+        return self.solve_cgmn(b, mode, tol, maxit, raise_on_notconv, method="crmn")
 
     def set_graphs(self, enable: bool):
         """Replay fixed kernel sequences from captured HIP graphs (default on)."""

@@ -102,3 +102,28 @@ def test_cgmn_matches_oracle(mode):
     assert abs(rel - info.relres) <= 1e-9 + 1e-6 * rel
     xf, _ = O.fgmres(v, wn, h3, cfg.nt, p6, b, tol=1e-10, maxinnerit=20)
     assert relerr(x, xf) <= 1e-4
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_crmn_matches_oracle(mode):
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small, extend, wavenumber, vec, point_source
+    cfg = config_small(n=(20, 18, 16), npml=4, freq=4.0)
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    b = point_source(cfg, cfg.sources[0])
+    h3, p6 = (cfg.h,) * 3, np.array(cfg.npml6)
+    v = "forw" if mode == 1 else "adj"
+    H = Helm3d(cfg.nt, h3, cfg.npml6, torch.from_numpy(wn).cuda())
+    print("crmn mode", mode, cfg.nt, flush=True)
+    x, info = H.solve_crmn(torch.from_numpy(b).cuda(), mode=mode, tol=1e-8, maxit=5000)
+    x = x.cpu().numpy()
+    sweep = lambda y, r: O.kacz_sweep(v, wn, h3, cfg.nt, p6, 1.5, y, r)  # noqa: E731
+    xo, its, hst = O.crmn(sweep, b, tol=1e-8, maxit=5000)
+    print("gpu crmn", info.iters, info.quasi_relres, info.relres, "oracle", its, hst[-1], flush=True)
+    assert info.converged and info.quasi_relres <= 1e-8
+    assert abs(info.iters - its) <= 0.02 * its + 2
+    assert relerr(x, xo) <= 1e-6
+    r = b - O.hmvp(v, wn, h3, cfg.nt, p6, x)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    assert abs(rel - info.relres) <= 1e-9 + 1e-6 * rel

@@ -92,3 +92,21 @@ def test_cgmn_converges_to_direct_solution(order):
     assert hst[-1] <= 1e-10
     assert np.linalg.norm(x - xs) <= 1e-7 * np.linalg.norm(xs)
     print(order, "cgmn iterations", its)
+
+
+@pytest.mark.parametrize("order", ["lex", "mc"])
+def test_crmn_converges_to_direct_solution(order):
+    """CRMN.m restatement: converges to the sparse direct solution, and its
+    residual history is monotone (conjugate residuals minimise ||r'|| over
+    the Krylov space of the Hermitian A')."""
+    wn = _wn(6)
+    A = F.assemble("forw", wn, H3, N3, NPML).tocsc()
+    b = np.zeros(A.shape[0], complex)
+    b[A.shape[0] // 2 + 5] = 1.0
+    xs = spla.splu(A).solve(b)
+    sweep = lambda x, r: O.kacz_sweep("forw", wn, H3, N3, NPML, 1.5, x, r, order=order)  # noqa: E731
+    x, its, hst = O.crmn(sweep, b, tol=1e-10, maxit=2000)
+    assert hst[-1] <= 1e-10
+    assert np.linalg.norm(x - xs) <= 1e-7 * np.linalg.norm(xs)
+    assert np.all(np.diff(hst) <= 1e-12 * hst[0])
+    print(order, "crmn iterations", its)
