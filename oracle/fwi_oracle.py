@@ -212,6 +212,28 @@ class Problem:
                 g += self.comp2phys @ np.real(F["Tadj"](U, V))
         return g
 
+    def hess(self, m, dm, D):
+        r"""HESS (PDEfunc.m:444-450) with the LS misfit (d2phi = I):
+        dU = Hf \ (-T(U) dm), V = Ha \ (-Pr' r), dV = Ha \ (-T(V) dm - Pr' Pr dU),
+        out += comp2phys Re(conj(U) .* (dm .* ddH' V) + conj(dU) .* dH' V + conj(U) .* dH' dV)
+        (discrete_helmholtz.m:286-293: T(u) dm = dH (dm .* u); DT_adj)."""
+        g = np.zeros(int(np.prod(self.n)))
+        h = self.d
+        for k, fr in enumerate(self.freq):
+            F = self._freq(m, fr)
+            vp = self.Pext @ np.asarray(m, dtype=np.float64).reshape(-1)
+            _, _, ddwn = O.param2wavenum(vp, fr, self.unit)
+            dmp = self.Pext @ np.asarray(dm, dtype=np.float64).reshape(-1)
+            for j in range(self.nsrc):
+                U = F["Hf"].solve(self._source(k, j))
+                _, r = O.ls_misfit(self.Pr @ U, D[:, k * self.nsrc + j])
+                dU = F["Hf"].solve(-F["T"](U, dm))
+                V = F["Ha"].solve(-(self.Pr.T @ r))
+                dV = F["Ha"].solve(-F["T"](V, dm) - self.Pr.T @ (self.Pr @ dU))
+                t1 = np.conj(U) * (dmp * O.hmvp("adj_deriv", ddwn, h, self.nt, self.npml, V))
+                g += self.comp2phys @ np.real(t1 + F["Tadj"](dU, V) + F["Tadj"](U, dV))
+        return g
+
     def hess_gn(self, m, dm):
         g = np.zeros(int(np.prod(self.n)))
         for k, fr in enumerate(self.freq):

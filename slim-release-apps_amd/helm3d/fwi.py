@@ -15,7 +15,8 @@ Mirrors tools/algorithms/CommonFreqModeling/PDEfunc.m for the 3D
 * ``OBJ`` (PDEfunc.m:383-394): f = sum 1/2 ||Pr U - D||^2 (LS_misfit.m:21-26),
   g = comp2phys sum Re(conj(U) .* dH^H V) with V = H^-H (-Pr^T (Pr U - D));
 * ``FORW_MODEL``, ``JACOB_FORW``, ``JACOB_ADJ``, ``HESS_GN``
-  (PDEfunc.m:421-444);
+  (PDEfunc.m:421-444) and the full Hessian-vector product ``HESS``
+  (PDEfunc.m:444-450, discrete_helmholtz.m:286-293);
 * ``zero_boundary`` (PDEfunc.m:478-482).
 
 Every solve, stencil, gradient, injection and sampling runs in the HIP
@@ -38,7 +39,8 @@ FORW_MODEL = "forw_model"
 JACOB_FORW = "jacob_forw"
 JACOB_ADJ = "jacob_adj"
 HESS_GN = "hess_gn"
-MODES = (OBJ, FORW_MODEL, JACOB_FORW, JACOB_ADJ, HESS_GN)
+HESS = "hess"
+MODES = (OBJ, FORW_MODEL, JACOB_FORW, JACOB_ADJ, HESS_GN, HESS)
 
 # Kaiser window parameter b by half-width r (opSincInterp.m:41-43); the
 # source and receiver operators use r = 4 (opInterp.m:34).
@@ -548,11 +550,54 @@ class FWIProblem:
         return g
 
 
+    def hess(self, v, dm, Dobs) -> torch.Tensor:
+        r"""PDEfunc(HESS) with the LS misfit (d2phi = I), PDEfunc.m:444-450:
+            dU = H \ (-T(U) dm),  V = H' \ (-Pr' dphi),
+            dV = H' \ (-T(V) dm - Pr' Pr dU),
+            out += comp2phys sum Re(DT_adj(U, dm, dU) V + T(U)' dV)
+        with T(u) dm = dH (dm .* u) and DT_adj(U, dm, dU) V = conj(U) .* (dm .*
+        ddH' V) + conj(dU) .* (dH' V) (discrete_helmholtz.m:286-293); dm is on
+        the extended grid (phys2comp).  T(V) dm is the forward derivative
+        operator applied to V, as the reference forms it."""
+        v = self._model_tensor(v)
+        dm = self._model_tensor(dm)
+        D = self._data(Dobs)
+        nsrc = self.Q.shape[1]
+        g = torch.zeros(int(np.prod(self.n)), dtype=torch.float64, device=self.device)
+        for k, freq in enumerate(self.model.freq):
+            S = self._setup(v, freq)
+            gpml = self._grad_buffers(S)
+            dwn_eff = self._dwn_times(S, freq, dm)
+            ddwn_dm = None
+            if self.model.unit == "m/s":  # ddwn = 0 for the slowness units (param2wavenum.m:36-49)
+                _, _, ddwn = self.H.param2wavenum(v, self.n, S["p6"], freq, self.model.unit, derivs=2)
+                ddwn_dm = ddwn
+                dm_pml = self.H.extend(dm, self.n, S["p6"])
+            for j in range(nsrc):
+                U = self._solve(self._source(S, k, j), 1)
+                r = self._sample(S, U) - D[:, k * nsrc + j]
+                dU = self._solve(-self.H.apply_deriv(dwn_eff, U), 1)
+                V = self._solve(self._inject_rec(S, r, -1.0), 2)
+                rhs = self._inject_rec(S, self._sample(S, dU), -1.0) - self.H.apply_deriv(dwn_eff, V)
+                dV = self._solve(rhs, 2)
+                self._grad_term(S, freq, dU, V, g, gpml)
+                self._grad_term(S, freq, U, dV, g, gpml)
+                if ddwn_dm is not None:
+                    # conj(U) .* (dm .* ddH' V) = conj(dm .* U) .* (ddH' V) for real dm
+                    Udm = U * dm_pml
+                    if gpml is None:
+                        self.H.grad_accum(Udm, V, g, dwn=ddwn_dm, grid="phys")
+                    else:
+                        self.H.grad_accum(Udm, V, gpml, dwn=ddwn_dm, grid="pml")
+            self._grad_finish(S, g, gpml)
+        return g
+
+
 def PDEfunc(func: str, v, Q, inp, Dobs, model: FWIModel, opts: Optional[FWIOpts] = None, problem=None):
     """PDEfunc(func, v, Q, input, Dobs, model, params) for the 3D operto27
     scheme.  Returns (f, g) for OBJ, the nrec x (nsrc nfreq) data for
     FORW_MODEL / JACOB_FORW, and the physical-grid image for JACOB_ADJ /
-    HESS_GN (float64 device tensors)."""
+    HESS_GN / HESS (float64 device tensors; HESS uses Dobs)."""
     if func not in MODES:
         raise ValueError("Must choose a preset function.")
     P = problem or FWIProblem(model, opts, Q=Q)
@@ -564,6 +609,8 @@ def PDEfunc(func: str, v, Q, inp, Dobs, model: FWIModel, opts: Optional[FWIOpts]
         return P.jacob_forw(v, inp)
     if func == JACOB_ADJ:
         return P.jacob_adj(v, inp)
+    if func == HESS:
+        return P.hess(v, inp, Dobs)
     return P.hess_gn(v, inp)
 
 

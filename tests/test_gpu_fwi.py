@@ -166,6 +166,27 @@ def test_pdefunc_modes_vs_oracle():
     assert abs(np.real(np.vdot(dd, jd)) - np.dot(dm, ja)) <= 1e-8 * abs(np.dot(dm, ja))
 
 
+@pytest.mark.parametrize("unit,cut_pml", [("s2/m2", False), ("s2/m2", True), ("m/s", False)])
+def test_hess_vs_oracle(unit, cut_pml):
+    """PDEfunc(HESS) at a model with a non-zero data residual: the two
+    second-order wavefields and the three gradient terms against the oracle."""
+    from helm3d.fwi import PDEfunc, HESS
+    gpu, ora = _pair(unit=unit, cut_pml=cut_pml, freq=(5.0,))
+    m_true, m0 = _models(unit)
+    Dobs = ora.forward(m_true)
+    rng = np.random.default_rng(17)
+    dm = rng.standard_normal(m0.size) * (m0.max() - m0.min())
+    h = PDEfunc(HESS, m0, None, dm, Dobs, gpu.model, problem=gpu).cpu().numpy()
+    ho = ora.hess(m0, dm, Dobs)
+    assert relerr(h, ho) <= 1e-8
+    # linear in the model perturbation (the reference's dV term uses the
+    # forward derivative T(V) dm, so the product is not symmetric in dm)
+    dm2 = rng.standard_normal(m0.size) * (m0.max() - m0.min())
+    h2 = PDEfunc(HESS, m0, None, dm2, Dobs, gpu.model, problem=gpu).cpu().numpy()
+    h12 = PDEfunc(HESS, m0, None, dm + 2.0 * dm2, Dobs, gpu.model, problem=gpu).cpu().numpy()
+    assert relerr(h12, h + 2.0 * h2) <= 1e-8
+
+
 def test_gradient_finite_differences_20_directions():
     """Central differences of the device objective on 20 seeded directions
     (solves at tol 1e-10), 24x22x20 physical grid + 5-point PML."""
