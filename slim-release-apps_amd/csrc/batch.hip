@@ -140,28 +140,26 @@ __global__ __launch_bounds__(KB) void bup_kernel(const VecItem<float>* __restric
 
 template <typename R>
 __global__ __launch_bounds__(KB) void bbicg_init_kernel(const VecItem<R>* __restrict__ items,
-                                                        const int* __restrict__ active, int bs, int rs, int rhs,
-                                                        int ps, int vs, int zs)
+                                                        const int* __restrict__ active, int bs, int ps, int vs,
+                                                        int ts, int zs)
 {
     const VecItem<R>* I = item_of(items, active);
     if (!I) return;
     __shared__ double red[KB / 64];
     double s2[1] = {0.0};
     const cpx<R>* b = slot(I, bs);
-    cpx<R>* r = slot(I, rs);
-    cpx<R>* rh = slot(I, rhs);
     cpx<R>* p = slot(I, ps);
     cpx<R>* v = slot(I, vs);
+    cpx<R>* tt = slot(I, ts);
     cpx<R>* z = slot(I, zs);
     long lo, hi;
     kb::range_plain(I->n, blockIdx.x, I->nblk, lo, hi);
     const cpx<R> zero{R(0), R(0)};
     for (long t = lo + threadIdx.x; t < hi; t += KB) {
         const cpx<R> bv = b[t];
-        r[t] = bv;
-        rh[t] = bv;
         p[t] = zero;
         v[t] = zero;
+        tt[t] = zero;
         z[t] = zero;
         s2[0] += double(bv.re) * double(bv.re) + double(bv.im) * double(bv.im);
     }
@@ -175,14 +173,31 @@ __global__ __launch_bounds__(KB) void bbicg_init_kernel(const VecItem<R>* __rest
 }
 
 template <typename R>
-__global__ __launch_bounds__(KB) void bbicg_xr_kernel(const VecItem<R>* __restrict__ items,
-                                                      const int* __restrict__ active, int ps, int ss, int ts,
-                                                      int rhs, int zs, int rs)
+__global__ __launch_bounds__(KB) void bbicg_fin_kernel(const VecItem<R>* __restrict__ items,
+                                                       const int* __restrict__ active, int ps, int rs, int vs,
+                                                       int zs)
 {
     const VecItem<R>* I = item_of(items, active);
     if (!I) return;
-    kb::bicg_xr_body<R>(slot(I, ps), slot(I, ss), slot(I, ts), slot(I, rhs), slot(I, zs), slot(I, rs), I->n,
-                        I->sc, I->part, blockIdx.x, I->nblk);
+    const double* sc = I->sc;
+    const double ar = sc[BS_ALPHA], ai = sc[BS_ALPHA + 1];
+    const double tn = sc[BS_TT];
+    const double wr = tn > 0.0 ? sc[BS_TS] / tn : 0.0, wi = tn > 0.0 ? sc[BS_TS + 1] / tn : 0.0;
+    const cpx<R>* p = slot(I, ps);
+    const cpx<R>* r = slot(I, rs);
+    const cpx<R>* v = slot(I, vs);
+    cpx<R>* z = slot(I, zs);
+    long lo, hi;
+    kb::range_plain(I->n, blockIdx.x, I->nblk, lo, hi);
+    for (long t = lo + threadIdx.x; t < hi; t += KB) {
+        const cpx<R> pv = p[t], rv = r[t], vv = v[t], zv = z[t];
+        // s = r - alpha v
+        const double sr = double(rv.re) - (ar * double(vv.re) - ai * double(vv.im));
+        const double si = double(rv.im) - (ar * double(vv.im) + ai * double(vv.re));
+        const double zr = double(zv.re) + ar * double(pv.re) - ai * double(pv.im) + wr * sr - wi * si;
+        const double zi = double(zv.im) + ar * double(pv.im) + ai * double(pv.re) + wr * si + wi * sr;
+        z[t] = cpx<R>{R(zr), R(zi)};
+    }
 }
 
 template <int K, typename Fn>
@@ -288,19 +303,18 @@ hipError_t launch_bup(const VecItem<float>* i32, const VecItem<double>* i64, con
 
 template <typename R>
 hipError_t launch_bbicg_init(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int bslot,
-                             int r, int rh, int p, int v, int z, hipStream_t s)
+                             int p, int v, int t, int z, hipStream_t s)
 {
-    hipLaunchKernelGGL((bbicg_init_kernel<R>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, bslot, r, rh,
-                       p, v, z);
+    hipLaunchKernelGGL((bbicg_init_kernel<R>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, bslot, p, v,
+                       t, z);
     return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_bbicg_xr(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int p, int sv,
-                           int t, int rh, int z, int r, hipStream_t s)
+hipError_t launch_bbicg_fin(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int p, int r,
+                            int v, int z, hipStream_t s)
 {
-    hipLaunchKernelGGL((bbicg_xr_kernel<R>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, p, sv, t, rh,
-                       z, r);
+    hipLaunchKernelGGL((bbicg_fin_kernel<R>), dim3(nblk_max, nitems), dim3(KB), 0, s, items, active, p, r, v, z);
     return hipGetLastError();
 }
 
@@ -312,9 +326,9 @@ hipError_t launch_bbicg_xr(const VecItem<R>* items, const int* active, int nitem
     template hipError_t launch_breduce<R>(const VecItem<R>*, const int*, int, int, int, int, bool, hipStream_t); \
     template hipError_t launch_bnorm2<R>(const VecItem<R>*, const int*, int, int, int, hipStream_t);             \
     template hipError_t launch_bbicg_init<R>(const VecItem<R>*, const int*, int, int, int, int, int, int, int,   \
-                                             int, hipStream_t);                                                  \
-    template hipError_t launch_bbicg_xr<R>(const VecItem<R>*, const int*, int, int, int, int, int, int, int, int, \
-                                           hipStream_t);
+                                             hipStream_t);                                                       \
+    template hipError_t launch_bbicg_fin<R>(const VecItem<R>*, const int*, int, int, int, int, int, int,         \
+                                            hipStream_t);
 HELM3D_BATCH_INST(double)
 HELM3D_BATCH_INST(float)
 

@@ -61,13 +61,16 @@ hipError_t launch_bdown_norm(const VecItem<double>* i64, const VecItem<float>* i
 // ws64[dst] = complex128(ws32[src])
 hipError_t launch_bup(const VecItem<float>* i32, const VecItem<double>* i64, const int* active, int nitems,
                       int nblk_max, int src, int dst, hipStream_t s);
-// BiCGStab start from z = 0: r = rhat = b_slot, p = v = z = 0 (slots given),
-// partials of ||b||^2 in columns 0 and 2 (column 1 zero): rho_0 and ||r_0||^2
+// BiCGStab start from z = 0 with r = rhat = b (the b slot serves as both):
+// p = v = t = z = 0 (slots given), partials of ||b||^2 in columns 0 and 2
+// (column 1 zero): rho_0 and ||r_0||^2
 template <typename R>
 hipError_t launch_bbicg_init(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int bslot,
-                             int r, int rh, int p, int v, int z, hipStream_t s);
+                             int p, int v, int t, int z, hipStream_t s);
+// Last z update of the fused iteration: z += alpha p + omega (r - alpha v)
+// with alpha and omega = (t, s) / ||t||^2 from the item's scalar slots
 template <typename R>
-hipError_t launch_bbicg_xr(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int p, int sv,
-                           int t, int rh, int z, int r, hipStream_t s);
+hipError_t launch_bbicg_fin(const VecItem<R>* items, const int* active, int nitems, int nblk_max, int p, int r,
+                            int v, int z, hipStream_t s);
 
 }  // namespace helm3d

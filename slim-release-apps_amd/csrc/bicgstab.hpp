@@ -13,9 +13,11 @@ constexpr int BS_RHO1 = 3;   // second triple
 constexpr int BS_RV = 6;     // (rhat, v)
 constexpr int BS_TS = 8;     // (t, s) re, im ; BS_TS + 2: ||t||^2
 constexpr int BS_TT = 10;
-constexpr int BS_ALPHA = 12;
-constexpr int BS_OMEGA = 14;
-constexpr int BS_SLOTS = 16;
+constexpr int BS_RS = 11;    // (rhat, s) re, im (fused batched form)
+constexpr int BS_RT = 13;    // (rhat, t) re, im (fused batched form)
+constexpr int BS_ALPHA = 15;
+constexpr int BS_OMEGA = 17;
+constexpr int BS_SLOTS = 20;
 
 // Up to two (a, b) vector pairs for the dot kernel.
 template <typename R>

@@ -30,12 +30,18 @@ struct StencilLaunch {
     // write-only sink of kSinkSlots values for steps/lanes with no output
     cpx<R>* sink;
     int tiles_x, tiles_y, kchunk;
-    // fused BiCGStab forms (EPI 3: f0 = r, f1 = p, f2 = v, o2 = new p,
-    // rh = rhat; EPI 4: f0 = r, f1 = v, o2 = s); scalars in sc
+    // fused BiCGStab forms, scalars in sc:
+    //   EPI 3: f0 = r, f1 = p, f2 = v, f3 = t of the previous iteration;
+    //          y = new v, o2 = new p, o3 = new r, zz = z (updated in place),
+    //          rh = rhat
+    //   EPI 4: f0 = r, f1 = v; y = t, rh = rhat
     const cpx<R>* f0;
     const cpx<R>* f1;
     const cpx<R>* f2;
+    const cpx<R>* f3;
     cpx<R>* o2;
+    cpx<R>* o3;
+    cpx<R>* zz;
     const cpx<R>* rh;
     double* sc;
     int rho_new, rho_old, first;

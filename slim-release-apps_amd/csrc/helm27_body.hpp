@@ -70,12 +70,17 @@ __device__ inline Cd cd_div(Cd a, Cd b)
 // as the halo slots of the next plane.
 // EPI: 0 stores y = alpha*H x, 1 stores y = alpha*H x + beta*b, 2 runs the
 // gradient epilogue (accumulates Re(conj(U) . out) into g).  3 and 4 are the
-// two operator applications of a BiCGStab iteration with the vector updates
-// around them fused in (scalars read from the device slots L.sc):
-//   3: x = r + beta (p - omega v) formed while staging, p <- x stored at the
-//      own point, y = v = H x, partial (rhat, v)
-//   4: x = s = r - alpha v formed while staging, s stored, y = t = H s,
-//      partials (t, s) and ||t||^2
+// two operator applications of a BiCGStab iteration with every vector update
+// fused in (scalars read from the device slots L.sc), so an iteration is two
+// launches and s is never stored:
+//   3: from r, p, v, t of the previous iteration and its alpha, omega:
+//      s = r - alpha v, r' = s - omega t, x = p' = r' + beta (p - omega v)
+//      formed while staging, z += alpha p + omega s, p' and r' stored at the
+//      own point, y = v' = H x, partial (rhat, v'); rho' = (rhat, s) -
+//      omega (rhat, t) and beta come from the partials of the previous
+//      EPI 4 launch
+//   4: x = s = r - alpha v formed while staging, y = t = H s, partials
+//      (t, s), ||t||^2, (rhat, s) and (rhat, t)
 // The body runs one block `lin` of a launch of `G` blocks described by L.
 template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI>
 __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int lin, const int G)
@@ -93,6 +98,11 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     __shared__ cpx<R> sm[(ADJ || kWnSep) ? 1 : 2][HY][HX];
     __shared__ R sw[kWnSep ? 2 : 1][HY][HX];
     __shared__ cpx<R> szl[kMaxChunk + 2 + PF], szh[kMaxChunk + 2 + PF];  // z PML slots k0 .. k1+PF+1
+    // EPI 3: r' and the z increment of the staged points, read back at the
+    // own point
+    constexpr int kE3 = EPI == 3 ? 1 : 0;
+    __shared__ cpx<R> sr[1 + kE3][1 + kE3 * (HY - 1)][1 + kE3 * (HX - 1)];
+    __shared__ cpx<R> sd[1 + kE3][1 + kE3 * (HY - 1)][1 + kE3 * (HX - 1)];
 
     const int nx = op.nx, ny = op.ny, nz = op.nz;
 
@@ -167,27 +177,47 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // real-wn loads (forward); with a constant wn they read a valid dummy address
     const R* wnp = op.wn ? op.wn : reinterpret_cast<const R*>((EPI == 3 || EPI == 4) ? L.f0 : L.x);
 
-    // block partial sums: ||y||^2 (EPI 0/1), (rhat, v) (EPI 3), (t, s) and
-    // ||t||^2 (EPI 4)
-    constexpr int NV = EPI == 3 ? 2 : (EPI == 4 ? 3 : 1);
+    // block partial sums: ||y||^2 (EPI 0/1), (rhat, v) (EPI 3), (t, s),
+    // ||t||^2, (rhat, s), (rhat, t) (EPI 4)
+    constexpr int NV = EPI == 3 ? 2 : (EPI == 4 ? 7 : 1);
     double acc[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) acc[q] = 0.0;
     const int sink_slot = int((long(lin) * NT + tid) % kSinkSlots);
     const int sink_slot2 = (sink_slot + kSinkSlots / 2) % kSinkSlots;
+    const int sink_slot3 = (sink_slot + kSinkSlots / 4) % kSinkSlots;
+    const int sink_slot4 = (sink_slot + 3 * (kSinkSlots / 4)) % kSinkSlots;
 
-    // BiCGStab coefficients of the staged combination x = r + ca*f1 + cb*f2
-    // (EPI 3: ca = beta, cb = -beta*omega; EPI 4: ca = -alpha, cb = 0)
-    cpx<R> ca = zero<R>(), cb = zero<R>();
+    // BiCGStab coefficients.  EPI 3: s = r + na v, r' = s + nw t,
+    // p' = r' + bt p + nbw v, dz = za p + zw s (na = -alpha, nw = -omega,
+    // bt = beta, nbw = -beta omega, za = alpha, zw = omega; all zero on the
+    // first iteration, whose p, v, t are zero and r = b).  EPI 4: s = r + ca v
+    // (ca = -alpha).
+    cpx<R> ca = zero<R>();
+    cpx<R> na = zero<R>(), nw = zero<R>(), bt = zero<R>(), nbw = zero<R>(), za = zero<R>(), zw = zero<R>();
     if constexpr (EPI == 3 || EPI == 4) {
-        const double* sc = L.sc;
+        double* sc = L.sc;
         if constexpr (EPI == 3) {
             if (!L.first) {
-                const Cd beta = cd_mul(cd_div(cd_ld(sc, L.rho_new), cd_ld(sc, L.rho_old)),
-                                       cd_div(cd_ld(sc, BS_ALPHA), cd_ld(sc, BS_OMEGA)));
-                const Cd bo = cd_mul(beta, cd_ld(sc, BS_OMEGA));
-                ca = cpx<R>{R(beta.re), R(beta.im)};
-                cb = cpx<R>{R(-bo.re), R(-bo.im)};
+                const Cd a = cd_ld(sc, BS_ALPHA);
+                const double tn = sc[BS_TT];
+                const Cd w = tn > 0.0 ? Cd{sc[BS_TS] / tn, sc[BS_TS + 1] / tn} : Cd{0.0, 0.0};
+                const Cd wrt = cd_mul(w, cd_ld(sc, BS_RT));
+                const Cd rho = Cd{sc[BS_RS] - wrt.re, sc[BS_RS + 1] - wrt.im};
+                const Cd beta = cd_mul(cd_div(rho, cd_ld(sc, L.rho_old)), cd_div(a, w));
+                const Cd bw = cd_mul(beta, w);
+                if (lin == 0 && tid == 0) {  // no block of this launch reads these slots
+                    sc[L.rho_new] = rho.re;
+                    sc[L.rho_new + 1] = rho.im;
+                    sc[BS_OMEGA] = w.re;
+                    sc[BS_OMEGA + 1] = w.im;
+                }
+                na = cpx<R>{R(-a.re), R(-a.im)};
+                nw = cpx<R>{R(-w.re), R(-w.im)};
+                bt = cpx<R>{R(beta.re), R(beta.im)};
+                nbw = cpx<R>{R(-bw.re), R(-bw.im)};
+                za = cpx<R>{R(a.re), R(a.im)};
+                zw = cpx<R>{R(w.re), R(w.im)};
             }
         } else {
             const Cd a = cd_div(cd_ld(sc, L.rho_new), cd_ld(sc, BS_RV));
@@ -199,6 +229,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         }
     }
     cpx<R> Xlast = zero<R>();  // EPI 3/4: staged x at the own point of the previous plane
+    cpx<R> Rlast = zero<R>(), Dlast = zero<R>();  // EPI 3: r' and dz there
     // output scale: alpha, or alpha / sqrt(*anrm) with the norm held on the device
     const R alpha = (EPI <= 1 && L.anrm) ? R(double(L.alpha) / sqrt(*L.anrm)) : L.alpha;
 
@@ -211,6 +242,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         cpx<R> x0, x1;  // slots (EPI 3/4: r)
         cpx<R> e0, e1;  // EPI 3: p, EPI 4: v at the slots
         cpx<R> f0, f1;  // EPI 3: v at the slots
+        cpx<R> g0, g1;  // EPI 3: t at the slots
+        cpx<R> zo;      // EPI 3: z at own point of plane q-1
         R w0, w1;       // forward, real wn at the slots
         cpx<R> c0, c1;  // forward, complex wn at the slots
         cpx<R> wc;      // adjoint: wn at own point of plane q-1
@@ -228,6 +261,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             if constexpr (EPI == 3) {
                 S.f0 = L.f2[base + off0];
                 S.f1 = L.f2[base + off1];
+                S.g0 = L.f3[base + off0];
+                S.g1 = L.f3[base + off1];
             }
         } else {
             S.x0 = L.x[base + off0];
@@ -248,7 +283,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             else S.wc = cpx<R>{wnp[own], R(0)};
         }
         if constexpr (EPI == 1) S.bb = L.b[own];
-        if constexpr (EPI == 3) S.bb = L.rh[own];
+        if constexpr (EPI == 3 || EPI == 4) S.bb = L.rh[own];
+        if constexpr (EPI == 3) S.zo = L.zz[own];
         if constexpr (EPI == 2) {
             S.gu = L.gu[own];
             const int qk = min(max(q - 1 - L.p0[2], 0), L.pn[2] - 1);
@@ -267,13 +303,20 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         const bool inz = q >= 0 && q < nz;
         const bool v0 = inz && in0, v1 = inz && in1;
         cpx<R> x0 = S.x0, x1 = S.x1;
-        if constexpr (EPI == 3 || EPI == 4) {
+        if constexpr (EPI == 3) {
+            const cpx<R> s0 = cfma(S.x0, na, S.f0), s1 = cfma(S.x1, na, S.f1);
+            const cpx<R> r0 = cfma(s0, nw, S.g0), r1 = cfma(s1, nw, S.g1);
+            x0 = cfma(cfma(r0, bt, S.e0), nbw, S.f0);
+            x1 = cfma(cfma(r1, bt, S.e1), nbw, S.f1);
+            sr[buf][s0y][s0x] = r0;
+            sd[buf][s0y][s0x] = cfma(cmul(za, S.e0), zw, s0);
+            if (has1) {
+                sr[buf][s1y][s1x] = r1;
+                sd[buf][s1y][s1x] = cfma(cmul(za, S.e1), zw, s1);
+            }
+        } else if constexpr (EPI == 4) {
             x0 = cfma(x0, ca, S.e0);
             x1 = cfma(x1, ca, S.e1);
-            if constexpr (EPI == 3) {
-                x0 = cfma(x0, cb, S.f0);
-                x1 = cfma(x1, cb, S.f1);
-            }
         }
         x0 = v0 ? x0 : zero<R>();
         x1 = v1 ? x1 : zero<R>();
@@ -297,7 +340,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         // own-point operands of plane q-1 leave the set before the refill
         cpx<R> wc = S.wc;
         if constexpr (ADJ && !WNC) wc.re = op.wn ? wc.re : op.wn_const;
-        const cpx<R> bb = S.bb, gu = S.gu;
+        const cpx<R> bb = S.bb, gu = S.gu, zo = S.zo;
         const double gv = S.gv;
         fetch(q + PF, S);  // unconditional: a fixed number of loads per step
         __syncthreads();
@@ -436,25 +479,39 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         // lanes without an output write to the sink instead (keeps the vmcnt
         // bookkeeping exact).
         const bool live = emit && active;
-        if constexpr (EPI == 3 || EPI == 4) {
-            const cpx<R> xo = su[buf][ty + 1][tx + 1];  // own point of plane q
-            cpx<R>* dy = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
-            cpx<R>* d2 = live ? &L.o2[long(kc) * plane + offc] : &L.sink[sink_slot2];
+        if constexpr (EPI == 3) {
+            const long od = long(kc) * plane + offc;
+            cpx<R>* dy = live ? &L.y[od] : &L.sink[sink_slot];
+            cpx<R>* d2 = live ? &L.o2[od] : &L.sink[sink_slot2];
+            cpx<R>* d3 = live ? &L.o3[od] : &L.sink[sink_slot3];
+            cpx<R>* d4 = live ? &L.zz[od] : &L.sink[sink_slot4];
             *dy = out;
             *d2 = Xlast;
+            *d3 = Rlast;
+            *d4 = zo + Dlast;
             if (live) {
-                if constexpr (EPI == 3) {
-                    // (rhat, v) += conj(rhat) v
-                    acc[0] += double(bb.re) * double(out.re) + double(bb.im) * double(out.im);
-                    acc[1] += double(bb.re) * double(out.im) - double(bb.im) * double(out.re);
-                } else {
-                    // (t, s) += conj(t) s ; ||t||^2
-                    acc[0] += double(out.re) * double(Xlast.re) + double(out.im) * double(Xlast.im);
-                    acc[1] += double(out.re) * double(Xlast.im) - double(out.im) * double(Xlast.re);
-                    acc[2] += double(out.re) * double(out.re) + double(out.im) * double(out.im);
-                }
+                // (rhat, v) += conj(rhat) v
+                acc[0] += double(bb.re) * double(out.re) + double(bb.im) * double(out.im);
+                acc[1] += double(bb.re) * double(out.im) - double(bb.im) * double(out.re);
             }
-            Xlast = xo;
+            Xlast = su[buf][ty + 1][tx + 1];  // own point of plane q
+            Rlast = sr[buf][ty + 1][tx + 1];
+            Dlast = sd[buf][ty + 1][tx + 1];
+        } else if constexpr (EPI == 4) {
+            cpx<R>* dy = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
+            *dy = out;
+            if (live) {
+                // (t, s) += conj(t) s ; ||t||^2 ; (rhat, s) ; (rhat, t)
+                const double tr = out.re, ti = out.im, sr_ = Xlast.re, si = Xlast.im, hr = bb.re, hi = bb.im;
+                acc[0] += tr * sr_ + ti * si;
+                acc[1] += tr * si - ti * sr_;
+                acc[2] += tr * tr + ti * ti;
+                acc[3] += hr * sr_ + hi * si;
+                acc[4] += hr * si - hi * sr_;
+                acc[5] += hr * tr + hi * ti;
+                acc[6] += hr * ti - hi * tr;
+            }
+            Xlast = su[buf][ty + 1][tx + 1];  // own point of plane q
         } else if constexpr (EPI == 2) {
             const int qk = kc - L.p0[2];
             const bool g = live && gcol && qk >= 0 && qk < L.pn[2];

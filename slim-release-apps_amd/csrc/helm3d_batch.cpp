@@ -36,7 +36,9 @@ constexpr int kNrm = 2 * kMaxBasis;               // small[kNrm + c]: ||u_c||^2
 constexpr int kSc = 2 * kMaxBasis;                // sc stride per item (BS_SLOTS used)
 
 // complex64 workspace slots of the inner BiCGStab
-enum : int { S32_B = 0, S32_Z, S32_R, S32_RH, S32_P0, S32_P1, S32_V0, S32_V1, S32_S, S32_T, S32_COUNT };
+// (B is also the shadow residual rhat and the residual before the first
+// iteration: the inner solve starts from z = 0 and never writes it)
+enum : int { S32_B = 0, S32_Z, S32_R0, S32_R1, S32_P0, S32_P1, S32_V0, S32_V1, S32_T, S32_COUNT };
 
 template <typename T>
 int upload(T* dev, const std::vector<T>& host, hipStream_t s)
@@ -291,6 +293,8 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
     std::vector<VecItem<float>> i32(n);
     std::vector<StencilLaunch<double>> st64(size_t(n) * (m + 1)), res64(n);
     std::vector<StencilLaunch<float>> k1(size_t(n) * 2), k2(size_t(n) * 2);
+    // first iteration: beta = alpha = omega = 0 and r = b
+    std::vector<StencilLaunch<float>> k1first(Lo.mixed ? n : 0);
     size_t poff = 0;
     for (int i = 0; i < n; ++i) {
         helm3d_ctx* c = ctxs[i];
@@ -331,35 +335,40 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
             J = VecItem<float>{S(0), ld, Lo.N[i], Lo.nblk[i], Lo.sblk[i], I.part, I.small, I.sc, I.b, I.x};
             for (int par = 0; par < 2; ++par) {
                 const int cur = par, nxt = par ^ 1;
+                // iteration i (parity cur = i & 1) reads r, p, v of
+                // iteration i-1 from the cur slots and t, and writes the
+                // nxt slots; z is updated in place
                 StencilLaunch<float> A = base_launch<float>(c->op32, Lo.grid[i], B->sink);
-                A.f0 = S(S32_R);
+                A.f0 = S(cur == 0 ? S32_R0 : S32_R1);
                 A.f1 = S(S32_P0 + cur);
                 A.f2 = S(S32_V0 + cur);
+                A.f3 = S(S32_T);
                 A.o2 = S(S32_P0 + nxt);
+                A.o3 = S(nxt == 0 ? S32_R0 : S32_R1);
+                A.zz = S(S32_Z);
                 A.y = S(S32_V0 + nxt);
-                A.rh = S(S32_RH);
+                A.rh = S(S32_B);
                 A.sc = I.sc;
                 A.rho_new = cur == 0 ? BS_RHO0 : BS_RHO1;
                 A.rho_old = cur == 0 ? BS_RHO1 : BS_RHO0;
                 A.norm_part = I.part;
                 k1[size_t(par) * n + i] = A;
+                if (par == 0) {
+                    A.first = 1;
+                    A.f0 = S(S32_B);
+                    k1first[i] = A;
+                }
                 StencilLaunch<float> Bk = base_launch<float>(c->op32, Lo.grid[i], B->sink);
-                Bk.f0 = S(S32_R);
+                Bk.f0 = S(nxt == 0 ? S32_R0 : S32_R1);
                 Bk.f1 = S(S32_V0 + nxt);
-                Bk.o2 = S(S32_S);
                 Bk.y = S(S32_T);
+                Bk.rh = S(S32_B);
                 Bk.sc = I.sc;
                 Bk.rho_new = cur == 0 ? BS_RHO0 : BS_RHO1;
                 Bk.norm_part = I.part;
                 k2[size_t(par) * n + i] = Bk;
             }
         }
-    }
-    // first iteration: beta = 0, carried by a copy of the parity-0 K1 items
-    std::vector<StencilLaunch<float>> k1first;
-    if (Lo.mixed) {
-        k1first.assign(k1.begin(), k1.begin() + n);
-        for (auto& L : k1first) L.first = 1;
     }
     int* active = B->act_host;
     for (int i = 0; i < n; ++i) active[i] = 1;
@@ -382,13 +391,15 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
     int* d_active = at<int>(B, Lo.o_active);
     const int nbm = Lo.nblk_max;
 
-    // ---- inner solve sequence (fixed iterations), all items
+    // ---- inner solve sequence (fixed iterations), all items: two stencil
+    // launches and two reductions per BiCGStab iteration (every vector
+    // update is fused into the stencil launches), then the last z update
     const long inner_its = std::max<long>(1, o.inner_maxit);
     auto inner_seq = [&](hipStream_t s) -> int {
-        HIPCHK(launch_bbicg_init<float>(d_i32, d_active, n, nbm, S32_B, S32_R, S32_RH, S32_P0, S32_V0, S32_Z, s));
+        HIPCHK(launch_bbicg_init<float>(d_i32, d_active, n, nbm, S32_B, S32_P0, S32_V0, S32_T, S32_Z, s));
         HIPCHK(launch_breduce<float>(d_i32, d_active, n, 3, BS_RHO0, BT_SC, false, s));
         for (long it = 0; it < inner_its; ++it) {
-            const int par = int(it & 1), nxt = par ^ 1;
+            const int par = int(it & 1);
             const StencilLaunch<float>* d1 =
                 it == 0 ? d_k1first : at<StencilLaunch<float>>(B, Lo.o_k1) + size_t(par) * n;
             HIPCHK(launch_stencil_batch<float>(Lo.variant, 3, d1, d_first, n, Lo.sblk_total, d_active, Lo.ty,
@@ -396,11 +407,11 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
             HIPCHK(launch_breduce<float>(d_i32, d_active, n, 2, BS_RV, BT_SC, true, s));
             HIPCHK(launch_stencil_batch<float>(Lo.variant, 4, at<StencilLaunch<float>>(B, Lo.o_k2) + size_t(par) * n,
                                                d_first, n, Lo.sblk_total, d_active, Lo.ty, Lo.pf, s));
-            HIPCHK(launch_breduce<float>(d_i32, d_active, n, 3, BS_TS, BT_SC, true, s));
-            HIPCHK(launch_bbicg_xr<float>(d_i32, d_active, n, nbm, S32_P0 + nxt, S32_S, S32_T, S32_RH, S32_Z, S32_R,
-                                          s));
-            HIPCHK(launch_breduce<float>(d_i32, d_active, n, 3, nxt == 0 ? BS_RHO0 : BS_RHO1, BT_SC, false, s));
+            HIPCHK(launch_breduce<float>(d_i32, d_active, n, 7, BS_TS, BT_SC, true, s));
         }
+        const int last = int(inner_its & 1);  // slots of the last iteration's p, r, v
+        HIPCHK(launch_bbicg_fin<float>(d_i32, d_active, n, nbm, S32_P0 + last, last == 0 ? S32_R0 : S32_R1,
+                                       S32_V0 + last, S32_Z, s));
         return HELM3D_OK;
     };
     auto run_inner = [&]() -> int {
