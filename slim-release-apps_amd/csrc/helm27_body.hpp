@@ -137,15 +137,18 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // factored form: six complex multipliers times real-weighted sums of the
     // plane's 3x3 neighbourhood (24 registers instead of 18 complex
     // coefficients).  Adjoint: side values of the neighbours, conjugated.
+    // the y multipliers are uniform over the wave (one row j): read them at a
+    // wave-uniform index so they live in scalar registers
+    const int jcw = min(jw, ny - 1);
     cpx<R> px0{}, px2{}, py0{}, py2{}, pxf{}, pyf{};
     if constexpr (!DERIV) {
         if constexpr (!ADJ) {
             px0 = op.xlo[ic + 1]; px2 = op.xhi[ic + 1]; pxf = px0 + px2;
-            py0 = op.ylo[jc + 1]; py2 = op.yhi[jc + 1]; pyf = py0 + py2;
+            py0 = op.ylo[jcw + 1]; py2 = op.yhi[jcw + 1]; pyf = py0 + py2;
         } else {
             // neighbour i-1 couples through its hi value, i+1 through its lo value
             px0 = conjc(op.xhi[ic]); px2 = conjc(op.xlo[ic + 2]); pxf = conjc(op.xlo[ic + 1] + op.xhi[ic + 1]);
-            py0 = conjc(op.yhi[jc]); py2 = conjc(op.ylo[jc + 2]); pyf = conjc(op.ylo[jc + 1] + op.yhi[jc + 1]);
+            py0 = conjc(op.yhi[jcw]); py2 = conjc(op.ylo[jcw + 2]); pyf = conjc(op.ylo[jcw + 1] + op.yhi[jcw + 1]);
         }
         // z weights of this chunk: slots k0 .. k1+1 (planes k0-1 .. k1) plus
         // the padding entries read by the trailing pipeline steps
@@ -408,34 +411,19 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                 // diagonal pairs.
                 const cpx<R> sx0 = u[0][0] + u[2][0], sx2 = u[0][2] + u[2][2];
                 const R cx = op.c[0], cy = op.c[1], wxy = op.xy, wxz = op.xz, wyz = op.yz, w3 = op.w3a2;
-                cpx<R> f0 = cx * u[0][1], f2 = cx * u[2][1], f3 = cy * u[1][0], f5 = cy * u[1][2];
-                f0 = rfma(f0, -wxy, sy0);
-                f2 = rfma(f2, -wxy, sy2);
-                f3 = rfma(f3, -wxy, sx0);
-                f5 = rfma(f5, -wxy, sx2);
-                cpx<R> fy = wxy * sx1, fx = wxy * sy1;
-                fy = rfma(fy, -cy, u[1][1]);
-                fx = rfma(fx, -cx, u[1][1]);
-                cpx<R> g0 = (-wxz) * u[0][1], g2 = (-wxz) * u[2][1], g3 = (-wyz) * u[1][0], g5 = (-wyz) * u[1][2];
-                g0 = rfma(g0, -w3, sy0);
-                g2 = rfma(g2, -w3, sy2);
-                g3 = rfma(g3, -w3, sx0);
-                g5 = rfma(g5, -w3, sx2);
-                cpx<R> gy = w3 * sx1, gx = w3 * sy1;
-                gy = rfma(gy, wyz, u[1][1]);
-                gx = rfma(gx, wxz, u[1][1]);
-                Fn = cmul(px0, f0);
-                Fn = cfma(Fn, px2, f2);
-                Fn = cfma(Fn, py0, f3);
-                Fn = cfma(Fn, py2, f5);
-                Fn = cfma(Fn, pyf, fy);
-                Fn = cfma(Fn, pxf, fx);
-                Gn = cmul(px0, g0);
-                Gn = cfma(Gn, px2, g2);
-                Gn = cfma(Gn, py0, g3);
-                Gn = cfma(Gn, py2, g5);
-                Gn = cfma(Gn, pyf, gy);
-                Gn = cfma(Gn, pxf, gx);
+                // accumulated multiplier by multiplier (few live temporaries)
+                Fn = cmul(px0, rfma(cx * u[0][1], -wxy, sy0));
+                Gn = cmul(px0, rfma((-wxz) * u[0][1], -w3, sy0));
+                Fn = cfma(Fn, px2, rfma(cx * u[2][1], -wxy, sy2));
+                Gn = cfma(Gn, px2, rfma((-wxz) * u[2][1], -w3, sy2));
+                Fn = cfma(Fn, py0, rfma(cy * u[1][0], -wxy, sx0));
+                Gn = cfma(Gn, py0, rfma((-wyz) * u[1][0], -w3, sx0));
+                Fn = cfma(Fn, py2, rfma(cy * u[1][2], -wxy, sx2));
+                Gn = cfma(Gn, py2, rfma((-wyz) * u[1][2], -w3, sx2));
+                Fn = cfma(Fn, pyf, rfma(wxy * sx1, -cy, u[1][1]));
+                Gn = cfma(Gn, pyf, rfma(w3 * sx1, wyz, u[1][1]));
+                Fn = cfma(Fn, pxf, rfma(wxy * sy1, -cx, u[1][1]));
+                Gn = cfma(Gn, pxf, rfma(w3 * sy1, wxz, u[1][1]));
             }
         }
 
