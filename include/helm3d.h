@@ -143,7 +143,7 @@ int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *
 int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch);
 int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *blocks);
 /* Autotuner of the stencil launch shape for this context's grid: times every
- * candidate shape (64 x {4, 8} column tiles, prefetch {1, 2}, z chunks of the
+ * candidate shape (64 x {4, 8} column tiles, prefetch {1, 2, 3, 4}, z chunks of the
  * whole column down to 8 planes) on the operator of `mode` (1: H, else H^H)
  * in `precision`, `reps` launches each with HIP events on the context's
  * stream, and keeps the fastest (results are identical for every shape).  No

@@ -772,7 +772,7 @@ int helm3d_autotune(helm3d_ctx* c, int mode, int precision, int reps, int* kchun
         if (kc >= 8 || d == 1) chunks.push_back(kc);
     }
     for (int ty : {4, 8}) {
-        for (int pf : {1, 2}) {
+        for (int pf : {1, 2, 3, 4}) {
             for (int kc : chunks) {
                 c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kc, ty, pf);
                 CHK(ensure_work(c));
