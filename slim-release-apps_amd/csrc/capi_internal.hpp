@@ -160,6 +160,10 @@ struct helm3d_ctx {
     size_t nrm_cap = 0;
     double* ring_host = nullptr;  // pinned: 2 rows
     hipEvent_t ring_ev[2] = {nullptr, nullptr};
+    // device addresses of the pinned mirrors: the reduction kernels write
+    // reduced values straight into them (no copy command per step)
+    double* small_host_dev = nullptr;
+    double* ring_dev = nullptr;
 
     Vecs<double> V64, Z64, T64, BC64;
     Vecs<float> V32, Z32, T32a, T32b, BC32;
@@ -246,12 +250,17 @@ inline int ensure_work(helm3d_ctx* c)
     CHK(ensure_t(&c->part, &c->part_cap, nblk * 2 * kMaxBasis));
     CHK(ensure_t(&c->small, &c->small_cap, 2 * kMaxBasis + 8));
     CHK(ensure(&c->sink, &c->sink_cap, size_t(kSinkSlots) * sizeof(cpx<double>)));
-    if (!c->small_host)
+    if (!c->small_host) {
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->small_host), (2 * kMaxBasis + 8) * sizeof(double)));
+        HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->small_host_dev), c->small_host, 0));
+    }
     CHK(ensure_t(&c->gs, &c->gs_cap, size_t(kMaxBasis + 1) * kGsRow));
     CHK(ensure_t(&c->nrm, &c->nrm_cap, size_t(kMaxBasis + 1)));
-    if (!c->ring_host)
+    if (!c->ring_host) {
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->ring_host), 2 * kGsRow * sizeof(double)));
+        HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->ring_dev), c->ring_host, 0));
+    }
+
     for (hipEvent_t& e : c->ring_ev)
         if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return HELM3D_OK;

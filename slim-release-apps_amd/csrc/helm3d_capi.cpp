@@ -203,13 +203,14 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
                 L.anrm = c->nrm + (j - 1);
                 L.sink = static_cast<cpx<R>*>(c->sink);
                 TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, c->stream));
+                // reduced dots into row (the update's coefficients) and the
+                // pinned ring row, ||slot(j)||^2 into nrm[j] and the ring row
+                double* hrow = c->ring_dev + (j & 1) * kGsRow;
                 TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, c->stream));
-                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, c->stream));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, c->stream, hrow));
                 TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk,
                                                            c->stream, c->nrm));
-                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, row + 2 * j, c->stream, c->nrm + j));
-                HIPCHK(hipMemcpyAsync(c->ring_host + (j & 1) * kGsRow, row, (2 * j + 1) * sizeof(double),
-                                      hipMemcpyDeviceToHost, c->stream));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, c->stream, hrow + 2 * j));
                 HIPCHK(hipEventRecord(c->ring_ev[j & 1], c->stream));
                 return HELM3D_OK;
             };
@@ -241,16 +242,15 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
                 CHK(P->apply(slot(k - 1), sig[k - 1], Z + size_t(k - 1) * ld));
                 CHK(stencil<R>(c, variant, op, Z + size_t(k - 1) * ld, slot(k), R(1), nullptr, R(0), nullptr));
             }
-            // classical Gram-Schmidt, FGMRES.m:100-103
-            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
-            CHK(reduce_to_small(c, nblk, 2 * k, 0));
+            // classical Gram-Schmidt, FGMRES.m:100-103; d (2k values) and
+            // ||w||^2 reach c->small and its pinned mirror
             CoefPack scl{};
             for (int q = 0; q < k; ++q) scl.v[q] = sig[q] * sig[q];
+            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
+            TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * k, c->small, c->stream, c->small_host_dev));
             TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, k, c->small, scl, slot(k), N, c->part, nblk, c->stream));
-            CHK(reduce_to_small(c, nblk, 1, 2 * k));
-            // fetch d (2k values) and ||w||^2 in one copy
-            HIPCHK(hipMemcpyAsync(c->small_host, c->small, (2 * k + 1) * sizeof(double), hipMemcpyDeviceToHost,
-                                  c->stream));
+            TIMED(HELM3D_PROF_REDUCE,
+                  launch_reduce(c->part, nblk, 1, c->small + 2 * k, c->stream, c->small_host_dev + 2 * k));
             HIPCHK(hipStreamSynchronize(c->stream));
             if (c->prof) CHK(prof_fold(c));
             for (int q = 0; q < k; ++q)
