@@ -314,7 +314,7 @@ def mg_sources_leg(plain_rhs_per_s, workers=4):
 def fwi_leg(solver="mg"):
     """One least-squares FWI gradient on the 256x256x128 workload; every
     solve to 1e-6 with FGMRES preconditioned by a 4-level complex64 V-cycle,
-    four solves at a time on four streams (solver "mg"), or with batched
+    six solves at a time on six streams (solver "mg"), or with batched
     mixed-precision solves ("mixed")."""
     import torch
     from helm3d.fwi import FWIOpts, FWIProblem
@@ -322,10 +322,10 @@ def fwi_leg(solver="mg"):
     model, m_true, m0 = fwi_c5()
     dev = torch.device("cuda:0")
     if solver == "mg":
-        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=4,
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=6,
                        mg_k=MG_K_TUNED)
-        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, 4 solves at a "
-               "time on 4 streams")
+        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, 6 solves at a "
+               "time on 6 streams")
     else:
         opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20)
         how = "batched mixed-precision solves (16 sources per batch)"
