@@ -291,7 +291,7 @@ def mg_sources_leg(plain_rhs_per_s, workers=4):
         b = torch.zeros(pool.H.N, dtype=torch.complex128, device=dev)
         b[(s[0] + p) + (s[1] + p) * nx + (s[2] + p) * nx * ny] = 1.0
         bs.append(b)
-    pool.solve_many(bs[:workers], precision="mixed")  # warm-up (workspaces)
+    pool.solve_many(bs, precision="mixed")  # warm-up: every hierarchy allocates its workspaces
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     xs, infos = pool.solve_many(bs, precision="mixed")
