@@ -95,6 +95,34 @@ def test_mixed_batch_uneven_grids(mode):
         assert err < 1e-3
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mixed_batch_complex_wavenumber(mode):
+    """Fused complex64 BiCGStab on the complex-wn operators (WNC variants):
+    attenuating wn = (1 + 0.05i) wn, residual checked with the oracle's wnc
+    stencil."""
+    import torch
+    from helm3d import BatchSolver, Helm3d
+    from helm3d.models import config_small, extend, wavenumber, vec
+    cfg = config_small(n=(32, 30, 24), npml=8, freq=5.0, seed=18)
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq)) * (1.0 + 0.05j)
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to("cuda:0"))
+    bs = [_rhs(cfg, s) for s in ((16, 15, 3), (5, 20, 12))]
+    print("mixed batch wnc mode", mode, flush=True)
+    S = BatchSolver("cuda:0")
+    xs, infos = S.solve([H, H], [torch.from_numpy(b).cuda() for b in bs], mode=mode, method="mixed", tol=1e-6,
+                        inner_maxit=10)
+    v = "forw_wnc" if mode == 1 else "adj_wnc"
+    for b, x, inf in zip(bs, xs, infos):
+        xn = x.cpu().numpy()
+        r = b - O.hmvp(v, wn, (cfg.h,) * 3, cfg.nt, np.array(cfg.npml6), xn)
+        rel = np.linalg.norm(r) / np.linalg.norm(b)
+        xr, _ = H.solve(torch.from_numpy(b).cuda(), mode=mode, tol=1e-10)
+        err = np.linalg.norm(xn - xr.cpu().numpy()) / np.linalg.norm(xr.cpu().numpy())
+        print("item iters", inf.iters, "relres", inf.relres, rel, "err", err)
+        assert inf.converged and rel < 1e-6
+        assert err < 1e-3
+
+
 def test_mixed_single_matches_unbatched_mixed():
     """A one-item batch (fused complex64 BiCGStab) and the single-vector mixed
     solve (unfused kernels) converge to the same solution."""
