@@ -152,14 +152,15 @@ struct helm3d_ctx {
     double* small_host = nullptr;  // pinned mirror
     // Arnoldi steps enqueued ahead of the host: per step k a row of
     // kGsRow doubles (dots d_0..d_{k-1}, then ||w||^2), the squared norms of
-    // the basis slots (scales of the update and of the next matvec), and a
-    // pinned two-row ring the rows are copied into (one event per row)
+    // the basis slots (scales of the update and of the next matvec), and
+    // pinned rows the reductions write into (one row and event per step of a
+    // cycle)
     double* gs = nullptr;  // device: (kMaxBasis + 1) rows
     size_t gs_cap = 0;
     double* nrm = nullptr;  // device: kMaxBasis + 1 values
     size_t nrm_cap = 0;
-    double* ring_host = nullptr;  // pinned: 2 rows
-    hipEvent_t ring_ev[2] = {nullptr, nullptr};
+    double* ring_host = nullptr;  // pinned: one row per step of a cycle
+    hipEvent_t ring_ev[kMaxBasis + 1] = {};
     // device addresses of the pinned mirrors: the reduction kernels write
     // reduced values straight into them (no copy command per step)
     double* small_host_dev = nullptr;
@@ -241,6 +242,9 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
 int model_variant(const helm3d_ctx* c, int mode_adj);
 
 constexpr int kGsRow = 2 * kMaxBasis + 2;  // doubles per Arnoldi-step row
+// grids of at most this many points enqueue a whole Arnoldi cycle ahead of
+// the host (gmres_engine without a preconditioner)
+constexpr long kDeepLookahead = 2L << 20;
 
 // Context work buffers: reduction partials, reduced values (+ pinned host
 // mirror) and the stencil's write-only sink.
@@ -257,7 +261,7 @@ inline int ensure_work(helm3d_ctx* c)
     CHK(ensure_t(&c->gs, &c->gs_cap, size_t(kMaxBasis + 1) * kGsRow));
     CHK(ensure_t(&c->nrm, &c->nrm_cap, size_t(kMaxBasis + 1)));
     if (!c->ring_host) {
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->ring_host), 2 * kGsRow * sizeof(double)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->ring_host), (kMaxBasis + 1) * kGsRow * sizeof(double)));
         HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->ring_dev), c->ring_host, 0));
     }
 

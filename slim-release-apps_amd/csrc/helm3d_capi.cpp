@@ -128,14 +128,17 @@ namespace helm3d {
 // unnormalised (slot c raw, scale sig[c]); P makes it the flexible variant of
 // FGMRES.m:96-98,118 with Z stored true-scaled.
 //
-// Without P the Arnoldi steps run one ahead of the host: the squared norms of
-// the basis slots stay on the device (c->nrm), where the next matvec takes its
-// input scale and the update pass its coefficients, so step k+1 is enqueued
-// before the host reads step k's dot products (copied into a pinned two-row
-// ring).  The host then builds the Hessenberg column and the quasi-residual
-// of step k while the device runs step k+1; a step enqueued past the stopping
-// point is discarded (its slot is never combined).  The arithmetic is that of
-// the one-step-at-a-time loop.
+// Without P the Arnoldi steps run ahead of the host: the squared norms of the
+// basis slots stay on the device (c->nrm), where the next matvec takes its
+// input scale and the update pass its coefficients, so steps are enqueued
+// before the host reads the dot products of earlier ones (written into pinned
+// rows, one per step).  On large grids one step runs ahead (the device is
+// busy anyway); on grids of at most kDeepLookahead points -- coarse multigrid
+// levels, where a step is a few short kernels -- the whole cycle is enqueued
+// at once and the host meets the device once per cycle instead of once per
+// step.  The host builds the Hessenberg columns and quasi-residuals in order;
+// steps enqueued past the stopping point are discarded (their slots are never
+// combined), so the arithmetic is that of the one-step-at-a-time loop.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
                  long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst)
@@ -205,22 +208,24 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
                 TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, c->stream));
                 // reduced dots into row (the update's coefficients) and the
                 // pinned ring row, ||slot(j)||^2 into nrm[j] and the ring row
-                double* hrow = c->ring_dev + (j & 1) * kGsRow;
+                double* hrow = c->ring_dev + size_t(j) * kGsRow;
                 TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, c->stream));
                 TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, c->stream, hrow));
                 TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk,
                                                            c->stream, c->nrm));
                 TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, c->stream, hrow + 2 * j));
-                HIPCHK(hipEventRecord(c->ring_ev[j & 1], c->stream));
+                HIPCHK(hipEventRecord(c->ring_ev[j], c->stream));
                 return HELM3D_OK;
             };
             // may a step follow once `done` steps have been taken (iteration cap)
             auto may_continue = [&](long done) { return iter_cap <= 0 || done < iter_cap; };
-            CHK(enqueue(1));
+            // steps in flight: the one the host waits for and those after it
+            const int depth = N <= kDeepLookahead ? m : 2;
+            int next = 1;  // next step of the cycle to enqueue
             while (k <= m && hlast > tol) {
-                if (k + 1 <= m && may_continue(inner + 1)) CHK(enqueue(k + 1));
-                HIPCHK(hipEventSynchronize(c->ring_ev[k & 1]));
-                const double* h = c->ring_host + (k & 1) * kGsRow;
+                while (next <= m && next < k + depth && may_continue(inner + (next - k))) CHK(enqueue(next++));
+                HIPCHK(hipEventSynchronize(c->ring_ev[k]));
+                const double* h = c->ring_host + size_t(k) * kGsRow;
                 for (int q = 0; q < k; ++q) H[q + (k - 1) * (m + 1)] = sig[q] * C(h[2 * q], h[2 * q + 1]);
                 const double hn = std::sqrt(h[2 * k]);
                 H[k + (k - 1) * (m + 1)] = hn;
