@@ -153,8 +153,10 @@ int helm3d_autotune(helm3d_ctx *ctx, int mode, int precision, int reps, int *kch
                     int *prefetch, double *best_ms);
 
 /* Context options.  HELM3D_OPT_GRAPHS (default 1): replay fixed kernel
- * sequences (the fixed-length inner solve of helm3d_solve_mixed) from captured
- * HIP graphs; 0 launches them one by one (identical results). */
+ * sequences from captured HIP graphs -- the fixed-length inner solve of
+ * helm3d_solve_mixed, and whole Arnoldi cycles of the unpreconditioned GMRES
+ * on grids of at most 2M points (coarse multigrid levels); 0 launches them one
+ * by one (identical results). */
 #define HELM3D_OPT_GRAPHS 1
 int helm3d_set_option(helm3d_ctx *ctx, int option, long long value);
 

@@ -171,6 +171,12 @@ struct helm3d_ctx {
     double* bsc = nullptr;  // BiCGStab device scalars (BS_SLOTS doubles)
     size_t bsc_cap = 0;
 
+    // captured Arnoldi cycle of the unpreconditioned GMRES on small grids,
+    // one per precision (0: complex128, 1: complex64), with the bytes of the
+    // launch arguments it was captured with (a different key recaptures)
+    hipGraphExec_t cyc_graph[2] = {nullptr, nullptr};
+    std::vector<unsigned char> cyc_key[2];
+
     // captured fixed-length complex64 inner solve (mixed precision)
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t inner_graph = nullptr;
@@ -199,6 +205,8 @@ struct helm3d_ctx {
             if (e) (void)hipEventDestroy(e);
         for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
         if (inner_graph) (void)hipGraphExecDestroy(inner_graph);
+        for (hipGraphExec_t g : cyc_graph)
+            if (g) (void)hipGraphExecDestroy(g);
         if (cap_stream) (void)hipStreamDestroy(cap_stream);
     }
 };

@@ -3,6 +3,8 @@
 // Host side of the hot path: operator contexts (device-resident model, PML
 // tables, Krylov workspace), the drop-in matvec of the MEX boundary, and the
 // restarted GMRES driver of FGMRES.m with its FP64 Hessenberg least squares.
+#include <type_traits>
+
 #include "capi_internal.hpp"
 
 namespace helm3d {
@@ -124,6 +126,50 @@ int fetch_small(helm3d_ctx* c, int count)
 
 namespace helm3d {
 
+// Launch the captured Arnoldi cycle of precision R on the context stream,
+// capturing it first (on the capture stream, through `record`) when no graph
+// exists for the current launch arguments: the operator descriptor, launch
+// shape, cycle length and every buffer address the steps use.
+template <typename R, typename Rec>
+int cycle_graph(helm3d_ctx* c, int variant, int m, const cpx<R>* V, long ld, int nblk, Rec&& record)
+{
+    const int p = std::is_same<R, double>::value ? 0 : 1;
+    std::vector<unsigned char> key;
+    auto put = [&](const void* a, size_t n) {
+        const auto* b = static_cast<const unsigned char*>(a);
+        key.insert(key.end(), b, b + n);
+    };
+    const OpDesc<R>& op = op_of<R>(c);
+    const void* ptrs[] = {V, c->part, c->nrm, c->gs, c->ring_dev, c->sink};
+    const long ints[] = {variant, m, ld, nblk, c->N, c->grid.tiles_x, c->grid.tiles_y, c->grid.kchunk,
+                         c->grid.blocks, c->grid.ty, c->grid.pf};
+    put(&op, sizeof(op));
+    put(ptrs, sizeof(ptrs));
+    put(ints, sizeof(ints));
+    if (!c->cyc_graph[p] || c->cyc_key[p] != key) {
+        if (c->cyc_graph[p]) {
+            HIPCHK(hipGraphExecDestroy(c->cyc_graph[p]));
+            c->cyc_graph[p] = nullptr;
+        }
+        if (!c->cap_stream) HIPCHK(hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+        const int st = record(c->cap_stream);
+        const hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+        if (st != HELM3D_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIPCHK(e);
+        const hipError_t ei = hipGraphInstantiate(&c->cyc_graph[p], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHK(ei);
+        c->cyc_key[p] = std::move(key);
+    }
+    HIPCHK(hipGraphLaunch(c->cyc_graph[p], c->stream));
+    return HELM3D_OK;
+}
+
 // Restarted GMRES of FGMRES.m:33-128 on precision R.  The basis is kept
 // unnormalised (slot c raw, scale sig[c]); P makes it the flexible variant of
 // FGMRES.m:96-98,118 with Z stored true-scaled.
@@ -195,7 +241,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         if (!P) {
             // enqueue Arnoldi step j: slot(j) = A slot(j-1)/||slot(j-1)||, dots into
             // row j, update with device scales, ||slot(j)||^2 into row j and nrm[j]
-            auto enqueue = [&](int j) -> int {
+            auto enqueue_on = [&](int j, hipStream_t s, bool record) -> int {
                 double* row = c->gs + size_t(j) * kGsRow;
                 StencilLaunch<R> L{};
                 L.op = op;
@@ -205,26 +251,38 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
                 L.beta = R(0);
                 L.anrm = c->nrm + (j - 1);
                 L.sink = static_cast<cpx<R>*>(c->sink);
-                TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, c->stream));
+                TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, s));
                 // reduced dots into row (the update's coefficients) and the
                 // pinned ring row, ||slot(j)||^2 into nrm[j] and the ring row
                 double* hrow = c->ring_dev + size_t(j) * kGsRow;
-                TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, c->stream));
-                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, c->stream, hrow));
-                TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk,
-                                                           c->stream, c->nrm));
-                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, c->stream, hrow + 2 * j));
-                HIPCHK(hipEventRecord(c->ring_ev[j], c->stream));
+                TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, s));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, s, hrow));
+                TIMED(HELM3D_PROF_UPDATE,
+                      launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk, s, c->nrm));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, s, hrow + 2 * j));
+                if (record) HIPCHK(hipEventRecord(c->ring_ev[j], s));
                 return HELM3D_OK;
             };
+            auto enqueue = [&](int j) { return enqueue_on(j, c->stream, true); };
             // may a step follow once `done` steps have been taken (iteration cap)
             auto may_continue = [&](long done) { return iter_cap <= 0 || done < iter_cap; };
             // steps in flight: the one the host waits for and those after it
             const int depth = N <= kDeepLookahead ? m : 2;
             int next = 1;  // next step of the cycle to enqueue
+            // a whole cycle ahead: replay it from a captured graph, one event
+            // after its last step
+            const bool graph = depth == m && c->use_graphs && !c->prof && may_continue(inner + m - 1);
+            if (graph) {
+                CHK(cycle_graph<R>(c, variant, m, V, ld, nblk, [&](hipStream_t s) -> int {
+                    for (int j = 1; j <= m; ++j) CHK(enqueue_on(j, s, false));
+                    return HELM3D_OK;
+                }));
+                HIPCHK(hipEventRecord(c->ring_ev[m], c->stream));
+                next = m + 1;
+            }
             while (k <= m && hlast > tol) {
                 while (next <= m && next < k + depth && may_continue(inner + (next - k))) CHK(enqueue(next++));
-                HIPCHK(hipEventSynchronize(c->ring_ev[k]));
+                HIPCHK(hipEventSynchronize(c->ring_ev[graph ? m : k]));
                 const double* h = c->ring_host + size_t(k) * kGsRow;
                 for (int q = 0; q < k; ++q) H[q + (k - 1) * (m + 1)] = sig[q] * C(h[2 * q], h[2 * q + 1]);
                 const double hn = std::sqrt(h[2 * k]);

@@ -122,6 +122,57 @@ def test_mixed_precision_solve(inner):
     assert ia.converged and np.linalg.norm(ra) / np.linalg.norm(b) < 1e-6
 
 
+@pytest.mark.parametrize("m", [20, 5])
+def test_captured_arnoldi_cycles_match_eager(m):
+    """On small grids a whole Arnoldi cycle is replayed from a captured HIP
+    graph: every cycle of every solve (eager residuals and combines between
+    the replays, forward and adjoint re-captures) gives the bits of the
+    step-by-step launches."""
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small
+    cfg = config_small(n=(30, 28, 24), npml=6)
+    wn, b = _setup(cfg)
+    dev = torch.device("cuda:0")
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    bt = torch.from_numpy(b).to(dev)
+    ref = {}
+    H.set_graphs(False)
+    for mode in (1, 2):
+        ref[mode] = H.solve(bt, mode=mode, tol=1e-8, maxinnerit=m)
+    H.set_graphs(True)
+    for rep in range(2):
+        for mode in (1, 2):
+            x, info = H.solve(bt, mode=mode, tol=1e-8, maxinnerit=m)
+            xe, ie = ref[mode]
+            print("mode", mode, "rep", rep, "iters", info.iters, ie.iters, "cycles", info.cycles, flush=True)
+            assert info.iters == ie.iters and info.cycles == ie.cycles and torch.equal(x, xe)
+            H.apply(bt)  # eager work between replays
+            torch.cuda.synchronize()
+    assert ref[1][1].cycles > 2  # several replays within one solve
+
+
+def test_multigrid_graphs_match_eager():
+    """Multigrid smoothers replay captured cycles on the coarse levels: the
+    solution equals the step-by-step one bit for bit."""
+    import torch
+    from helm3d.mg import MLGMRES
+    from helm3d.models import config_small, point_source, vec
+    cfg = config_small(n=(40, 36, 30), npml=8)
+    dev = torch.device("cuda:0")
+    M = MLGMRES(vec(cfg.v), cfg.n_phys, cfg.h, cfg.npml, cfg.freq, nlevels=3, device=dev, autotune=False)
+    b = torch.from_numpy(point_source(cfg, cfg.sources[0])).to(dev)
+    out = {}
+    for g in (False, True, True):
+        for H in M.ops + M.smoothers:
+            H.set_graphs(g)
+        x, info = M.solve(b, tol=1e-8, precision="mixed")
+        if g in out:
+            assert torch.equal(x, out[g][0])
+        out[g] = (x, info)
+    assert out[True][1].iters == out[False][1].iters and torch.equal(out[True][0], out[False][0])
+
+
 def test_mixed_fixed_inner_graph_replays_match_eager():
     """Fixed-length complex64 inner solves replayed from a captured HIP graph
     give bit-identical solutions to eager launches, over repeated replays with
