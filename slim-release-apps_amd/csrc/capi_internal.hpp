@@ -242,10 +242,14 @@ struct GmresResult {
 };
 
 // Restarted GMRES of FGMRES.m:33-128 on the context's operator (variant) and
-// workspace (helm3d_capi.cpp).  P makes it flexible.
+// workspace (helm3d_capi.cpp).  P makes it flexible.  need_result = false is
+// for callers that use only x (multigrid smoothers and coarse solves): the
+// true residual after the last cycle (which decides nothing) is not formed and
+// res->relres holds the quasi-residual; x is the same.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
-                 long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst);
+                 long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst,
+                 bool need_result = true);
 
 int model_variant(const helm3d_ctx* c, int mode_adj);
 

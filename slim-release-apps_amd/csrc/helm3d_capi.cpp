@@ -187,7 +187,8 @@ int cycle_graph(helm3d_ctx* c, int variant, int m, const cpx<R>* V, long ld, int
 // combined), so the arithmetic is that of the one-step-at-a-time loop.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
-                 long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst)
+                 long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst,
+                 bool need_result)
 {
     using C = std::complex<double>;
     if (m < 1 || m > kMaxBasis - 1) return fail(HELM3D_ERR_ARG, "maxinnerit must be in [1, 31]");
@@ -207,28 +208,41 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     const int nblk = krylov_blocks(N);
     auto slot = [&](int s) { return V + size_t(s) * ld; };
 
-    // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62); ||r0||^2 also into nrm[0]
+    // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62); ||r0||^2 into nrm[0]
+    // and the pinned mirror
     if (use_x0) {
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
-        CHK(reduce_to_small(c, c->grid.blocks, 1, 0, c->nrm));
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, c->grid.blocks, 1, c->nrm, c->stream, c->small_host_dev));
     } else {
         HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, slot(0), N, false, c->stream)));
         HIPCHK((launch_axpby<R, R>(0.0, 0.0, static_cast<const cpx<R>*>(nullptr), 0.0, 0.0, x, N, false, c->stream)));
         HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
-        CHK(reduce_to_small(c, nblk, 1, 0, c->nrm));
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm, c->stream, c->small_host_dev));
     }
-    CHK(fetch_small(c, 1));
-    const double normr0 = std::sqrt(c->small_host[0]);
+    // Without P, a whole cycle ahead and no result wanted, ||r0|| is read
+    // with the cycle's first row (one host round trip less): the cycle needs
+    // it only on the device, and with r0 = 0 it is discarded.
+    bool defer_norm = !P && !need_result && N <= kDeepLookahead && maxit >= 1 && tol < 1.0;
+    double normr0 = 0.0, beta = 0.0;
     double hlast = 1.0;
     if (hst) hst->push_back(1.0);
-    if (normr0 == 0.0) {
+    auto take_norm = [&]() {
+        normr0 = std::sqrt(c->small_host[0]);
+        beta = normr0;
+    };
+    auto zero_residual = [&]() {
         // zero residual: the solution is x (opAbstractMatrix.m:118 returns zeros for b = 0)
         res->relres = 0.0;
         res->quasi = 0.0;
         res->converged = true;
         return HELM3D_OK;
+    };
+    if (!defer_norm) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->prof) CHK(prof_fold(c));
+        take_norm();
+        if (normr0 == 0.0) return zero_residual();
     }
-    double beta = normr0;
     long it = 1, inner = 0;
     std::vector<C> H(size_t(m + 1) * m), y;
     std::vector<double> sig(m + 1, 0.0);
@@ -236,7 +250,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
 
     while (hlast > tol && (it / m) < maxit) {
         std::fill(H.begin(), H.end(), C(0));
-        sig[0] = 1.0 / beta;
+        sig[0] = defer_norm ? 0.0 : 1.0 / beta;
         int k = 1;
         if (!P) {
             // enqueue Arnoldi step j: slot(j) = A slot(j-1)/||slot(j-1)||, dots into
@@ -283,6 +297,12 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             while (k <= m && hlast > tol) {
                 while (next <= m && next < k + depth && may_continue(inner + (next - k))) CHK(enqueue(next++));
                 HIPCHK(hipEventSynchronize(c->ring_ev[graph ? m : k]));
+                if (defer_norm) {
+                    take_norm();
+                    if (normr0 == 0.0) return zero_residual();
+                    sig[0] = 1.0 / beta;
+                    defer_norm = false;
+                }
                 const double* h = c->ring_host + size_t(k) * kGsRow;
                 for (int q = 0; q < k; ++q) H[q + (k - 1) * (m + 1)] = sig[q] * C(h[2 * q], h[2 * q + 1]);
                 const double hn = std::sqrt(h[2 * k]);
@@ -343,6 +363,14 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         } else {
             TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(V, ld, k - 1, a, x, N, nblk, c->stream));
         }
+        // no further cycle (quasi-residual below tol, maxit or the iteration
+        // cap): the true residual decides nothing, so a caller that uses only
+        // x skips it
+        if (!need_result && (hlast <= tol || (it / m) >= maxit || capped)) {
+            res->relres = hlast;
+            ++res->cycles;
+            break;
+        }
         // true residual r = b - A x into slot 0, FGMRES.m:120-124
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
         CHK(reduce_to_small(c, c->grid.blocks, 1, 0, c->nrm));
@@ -361,9 +389,9 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
 }
 
 template int gmres_engine<double>(helm3d_ctx*, int, const cpx<double>*, cpx<double>*, bool, double, int, long, long,
-                                  PrecondT<double>*, GmresResult*, std::vector<double>*);
+                                  PrecondT<double>*, GmresResult*, std::vector<double>*, bool);
 template int gmres_engine<float>(helm3d_ctx*, int, const cpx<float>*, cpx<float>*, bool, double, int, long, long,
-                                 PrecondT<float>*, GmresResult*, std::vector<double>*);
+                                 PrecondT<float>*, GmresResult*, std::vector<double>*, bool);
 
 int model_variant(const helm3d_ctx* c, int mode_adj)
 {

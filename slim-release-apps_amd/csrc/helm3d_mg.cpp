@@ -143,7 +143,7 @@ struct CoarseInner : PrecondT<R> {
         HIPCHK((launch_axpby<R, R>(s, 0.0, v, 0.0, 0.0, as<R>(L.vb), L.N, false, mg->stream)));
         GmresResult r;
         return gmres_engine<R>(L.S, model_variant(L.S, mode != 1), as<R>(L.vb), z, false, k->coarse_tol,
-                               k->kc_inner, k->kc_outer, 0, nullptr, &r, nullptr);
+                               k->kc_inner, k->kc_outer, 0, nullptr, &r, nullptr, false);
     }
 };
 
@@ -191,7 +191,7 @@ int vcycle(helm3d_mg* mg, const MgParams& k, int l, int mode, const cpx<R>* b, c
     GmresResult r;
     // pre-smoothing from zero
     CHK(gmres_engine<R>(L.S, model_variant(L.S, mode != 1), b, x, false, k.smooth_tol, k.ks_inner, k.ks_outer, 0,
-                        nullptr, &r, nullptr));
+                        nullptr, &r, nullptr, false));
     CHK(residual<R>(L, mode, b, x, as<R>(L.vr)));
     HIPCHK(launch_restrict<R>(L.T, 0.125, as<R>(L.vr), as<R>(C.crhs), mg->stream));
     // coarse-grid solve, preconditioned by the next level
@@ -202,19 +202,19 @@ int vcycle(helm3d_mg* mg, const MgParams& k, int l, int mode, const cpx<R>* b, c
         P.l = l + 1;
         P.mode = mode;
         CHK(gmres_engine<R>(C.A, model_variant(C.A, mode != 1), as<R>(C.crhs), as<R>(C.cx), false, k.coarse_tol,
-                            k.kc_inner, k.kc_outer, 0, &P, &r, nullptr));
+                            k.kc_inner, k.kc_outer, 0, &P, &r, nullptr, false));
     } else {
         CoarseInner<R> P;
         P.mg = mg;
         P.k = &k;
         P.mode = mode;
         CHK(gmres_engine<R>(C.A, model_variant(C.A, mode != 1), as<R>(C.crhs), as<R>(C.cx), false, k.coarse_tol,
-                            k.kc_inner, k.kc_outer, 0, &P, &r, nullptr));
+                            k.kc_inner, k.kc_outer, 0, &P, &r, nullptr, false));
     }
     HIPCHK(launch_prolong_add<R>(L.T, as<R>(C.cx), x, mg->stream));
     // post-smoothing from the corrected x
     CHK(gmres_engine<R>(L.S, model_variant(L.S, mode != 1), b, x, true, k.smooth_tol, k.ks_inner, k.ks_outer, 0,
-                        nullptr, &r, nullptr));
+                        nullptr, &r, nullptr, false));
     return HELM3D_OK;
 }
 
