@@ -219,10 +219,10 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
         TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm, c->stream, c->small_host_dev));
     }
-    // Without P, a whole cycle ahead and no result wanted, ||r0|| is read
-    // with the cycle's first row (one host round trip less): the cycle needs
-    // it only on the device, and with r0 = 0 it is discarded.
-    bool defer_norm = !P && !need_result && N <= kDeepLookahead && maxit >= 1 && tol < 1.0;
+    // Without P and with no result wanted, ||r0|| is read with the first
+    // step's row (one host round trip less): the steps need it only on the
+    // device, and with r0 = 0 they are discarded.
+    bool defer_norm = !P && !need_result && maxit >= 1 && tol < 1.0;
     double normr0 = 0.0, beta = 0.0;
     double hlast = 1.0;
     if (hst) hst->push_back(1.0);
