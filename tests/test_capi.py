@@ -55,3 +55,46 @@ def test_error_paths_without_gpu():
     code = L.helm3d_mvp(0, P(z), None, P(np.ones(3)), P(n3), 3, P(np.zeros(6)), P(z), P(z), P(z), None, 7, 1)
     assert code == HELM3D_ERR_XSIZE
     assert b"x must have" in L.helm3d_last_error()
+
+
+@pytest.mark.parametrize("n", [(0.0, 2.0, 2.0), (2.5, 2.0, 2.0), (-1.0, 2.0, 2.0), (float("nan"), 2.0, 2.0)])
+def test_nsize_rejects_non_positive_and_fractional_sizes(n):
+    """n must hold three positive integers (Helm3dmvp_mex.c:75-79 casts them);
+    the check comes before any device call."""
+    import numpy as np
+    import helm3d
+    from helm3d._lib import HELM3D_ERR_NSIZE
+    L = helm3d.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    z = np.zeros(8)
+    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    code = L.helm3d_mvp(0, P(z), None, P(np.ones(3)), P(np.array(n)), 3, P(np.zeros(6)), P(z), P(z), P(z), None,
+                        8, 1)
+    assert code == HELM3D_ERR_NSIZE
+
+
+def test_argument_errors_without_gpu():
+    """Bad variant, null operands and negative PML widths are argument errors;
+    the sweep checks the rows of b (Helm3dkaczswp_mex.c Bsize)."""
+    import numpy as np
+    import helm3d
+    from helm3d._lib import HELM3D_ERR_ARG, HELM3D_ERR_BSIZE, HELM3D_ERR_XSIZE
+    L = helm3d.lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    z = np.zeros(8)
+    P = lambda a: a.ctypes.data_as(dp)  # noqa: E731
+    n3 = np.array([2.0, 2.0, 2.0])
+    h = np.ones(3)
+    assert L.helm3d_mvp(9, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), P(z), P(z), P(z), None, 8, 1) == HELM3D_ERR_ARG
+    assert L.helm3d_mvp(0, None, None, P(h), P(n3), 3, P(np.zeros(6)), P(z), P(z), P(z), None, 8, 1) == HELM3D_ERR_ARG
+    bad = np.array([1.0, -1.0, 0.0, 0.0, 0.0, 0.0])
+    assert L.helm3d_mvp(0, P(z), None, P(h), P(n3), 3, P(bad), P(z), P(z), P(z), None, 8, 1) == HELM3D_ERR_ARG
+    # complex-wn variant without the imaginary part
+    assert L.helm3d_mvp(4, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), P(z), P(z), P(z), None, 8, 1) == HELM3D_ERR_ARG
+    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 8, P(z), None, 7,
+                            P(z), P(z))
+    assert code == HELM3D_ERR_BSIZE
+    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 6, P(z), None, 8,
+                            P(z), P(z))
+    assert code == HELM3D_ERR_XSIZE
+
