@@ -122,7 +122,7 @@ def cpu_baseline(cfg, wn, b, gpu_iters, sample_iters):
     }
 
 
-def mixed_leg(plain_rhs_per_s):
+def mixed_leg(plain_rhs_per_s, restart=None):
     """8 sources on the benchmark model, batched mixed precision, versus the
     complex128 solutions of the same sources."""
     import torch
@@ -135,7 +135,7 @@ def mixed_leg(plain_rhs_per_s):
     bs = [torch.from_numpy(point_source(cfg, s)).to(dev) for s in cfg.sources]
     S = BatchSolver(dev)
     tuned = S.autotune([H] * len(bs), bs, inner_maxit=20)  # launch shape of the batched kernels
-    kw = dict(method="mixed", tol=1e-6, maxinnerit=20, inner_maxit=20)
+    kw = dict(method="mixed", tol=1e-6, maxinnerit=restart or MIXED_RESTART, inner_maxit=20)
     S.solve([H] * len(bs), bs, **kw)  # warm-up (allocation, graph capture)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -150,12 +150,12 @@ def mixed_leg(plain_rhs_per_s):
     return {"workload": "8 point sources (z=5, y=100, x=round(linspace(20,179,8))) on the 240x240x140 benchmark "
                         "model, one batched mixed-precision solve to 1e-6 FP64 residual",
             "rhs_per_s": rate, "seconds": dt, "vs_plain": rate / plain_rhs_per_s if plain_rhs_per_s else None,
-            "outer_iters": [i.iters for i in infos], "inner_iters_per_outer": 20,
+            "outer_iters": [i.iters for i in infos], "outer_restart": kw["maxinnerit"], "inner_iters_per_outer": 20,
             "final_relres": [i.relres for i in infos], "err_vs_complex128": errs, "launch": tuned,
             "dtype": "c64 inner / f64 outer"}
 
 
-def uneven_leg(plain_rhs_per_s):
+def uneven_leg(plain_rhs_per_s, restart=None):
     """The uneven batched workload: 8 subproblems (grids between 96^3 and
     256x256x128 + 20-pt PML, 3-8 Hz, 4-32 impulses each), every right-hand
     side solved to 1e-6 FP64 residual with batched mixed precision, split into
@@ -180,7 +180,7 @@ def uneven_leg(plain_rhs_per_s):
     first = [owner.index(k) for k in range(len(subs))]  # one right-hand side of every subproblem
     tuned = S.autotune([ops[i] for i in first], [bs[i] for i in first], inner_maxit=20, tile_rows=(4, 8),
                        prefetch=(2,), divisors=(1, 2))
-    kw = dict(method="mixed", tol=1e-6, maxinnerit=20, inner_maxit=20)
+    kw = dict(method="mixed", tol=1e-6, maxinnerit=restart or MIXED_RESTART, inner_maxit=20)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     xs, infos, chunks = S.solve_chunked(ops, bs, **kw)
@@ -225,6 +225,7 @@ def uneven_leg(plain_rhs_per_s):
 
 
 MG_K_DEFAULT = (3, 5, 3, 5)  # ML_GMRES.m:3-6
+MIXED_RESTART = 8            # outer FGMRES restart of the batched mixed-precision legs (8, 10, 15, 20, 30 swept)
 MG_K_TUNED = (1, 5, 3, 5)    # one smoothing cycle: fastest in a sweep of k on the 240x240x140 solve
 
 
@@ -429,6 +430,8 @@ def main():
     ap.add_argument("--launch", default=None,
                     help="--mode matvec only: stencil launch shape 'kchunk,tile_rows,prefetch' (no autotuning)")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
+    ap.add_argument("--mixed-restart", type=int, default=None,
+                    help="outer FGMRES restart of the batched mixed-precision legs (default MIXED_RESTART)")
     ap.add_argument("--mg-config", default=None,
                     help="--mode mg only: one 'levels,precision[,k1,k2,k3,k4]' set (e.g. 4,mixed or "
                          "4,mixed,1,5,3,5) instead of the default list")
@@ -456,9 +459,9 @@ def main():
     if args.mode in ("mixed", "uneven", "fwi", "mg", "mg-sources"):
         plain_rhs_per_s = None  # no plain solve in this run: ratios are left out
         if args.mode == "mixed":
-            res = mixed_leg(plain_rhs_per_s)
+            res = mixed_leg(plain_rhs_per_s, args.mixed_restart)
         elif args.mode == "uneven":
-            res = uneven_leg(plain_rhs_per_s)
+            res = uneven_leg(plain_rhs_per_s, args.mixed_restart)
         elif args.mode == "mg-sources":
             res = mg_sources_leg(plain_rhs_per_s)
         elif args.mode == "mg":
@@ -573,7 +576,7 @@ def main():
     del H
     torch.cuda.empty_cache()
     if not args.no_mixed:
-        out["mixed_batched"] = mixed_leg(args.steps / elapsed)
+        out["mixed_batched"] = mixed_leg(args.steps / elapsed, args.mixed_restart)
         print(f"[bench] mixed leg done: {out['mixed_batched']['rhs_per_s']:.3f} RHS/s", file=sys.stderr, flush=True)
     if not args.no_mg:
         out["multigrid"] = mg_leg(args.steps / elapsed)
@@ -584,7 +587,7 @@ def main():
               file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_uneven:
-        out["batched_uneven"] = uneven_leg(args.steps / elapsed)
+        out["batched_uneven"] = uneven_leg(args.steps / elapsed, args.mixed_restart)
         print(f"[bench] uneven batched leg done: {out['batched_uneven']['rhs_per_s']:.3f} RHS/s", file=sys.stderr,
               flush=True)
         torch.cuda.empty_cache()
