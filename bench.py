@@ -226,7 +226,7 @@ def uneven_leg(plain_rhs_per_s, restart=None):
 
 MG_K_DEFAULT = (3, 5, 3, 5)  # ML_GMRES.m:3-6
 MIXED_RESTART = 8            # outer FGMRES restart of the batched mixed-precision legs (8, 10, 15, 20, 30 swept)
-MG_K_TUNED = (1, 5, 3, 5)    # one smoothing cycle: fastest in a sweep of k on the 240x240x140 solve
+MG_K_TUNED = (1, 2, 3, 5)    # one smoothing cycle of 2 steps: fastest in sweeps of k on the 240x240x140 solve and the FWI gradient
 
 
 def mg_leg(plain_rhs_per_s, solves=3, configs=((3, "c128", MG_K_DEFAULT), (3, "mixed", MG_K_DEFAULT),
