@@ -315,7 +315,7 @@ def mg_sources_leg(plain_rhs_per_s, workers=4):
 def fwi_leg(solver="mg"):
     """One least-squares FWI gradient on the 256x256x128 workload; every
     solve to 1e-6 with FGMRES preconditioned by a 4-level complex64 V-cycle,
-    six solves at a time on six streams (solver "mg"), or with batched
+    eight solves at a time on eight streams (solver "mg"), or with batched
     mixed-precision solves ("mixed")."""
     import torch
     from helm3d.fwi import FWIOpts, FWIProblem
@@ -323,10 +323,11 @@ def fwi_leg(solver="mg"):
     model, m_true, m0 = fwi_c5()
     dev = torch.device("cuda:0")
     if solver == "mg":
-        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=6,
+        # 8 hierarchies: the 16 sources of a frequency split evenly
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=8,
                        mg_k=MG_K_TUNED)
-        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, 6 solves at a "
-               "time on 6 streams")
+        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, 8 solves at a "
+               "time on 8 streams")
     else:
         opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20)
         how = "batched mixed-precision solves (16 sources per batch)"
