@@ -269,10 +269,18 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
                 // reduced dots into row (the update's coefficients) and the
                 // pinned ring row, ||slot(j)||^2 into nrm[j] and the ring row
                 double* hrow = c->ring_dev + size_t(j) * kGsRow;
-                TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, s));
-                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, s, hrow));
-                TIMED(HELM3D_PROF_UPDATE,
-                      launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk, s, c->nrm));
+                if (nblk <= kInBlockDots) {
+                    // short dots columns: summed by every block of the update
+                    // pass (partials one column up, clear of the norm partials)
+                    TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part + nblk, nblk, s));
+                    TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, nullptr, CoefPack{}, slot(j), N, c->part,
+                                                               nblk, s, c->nrm, c->part + nblk, hrow));
+                } else {
+                    TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, s));
+                    TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, s, hrow));
+                    TIMED(HELM3D_PROF_UPDATE,
+                          launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk, s, c->nrm));
+                }
                 TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, s, hrow + 2 * j));
                 if (record) HIPCHK(hipEventRecord(c->ring_ev[j], s));
                 return HELM3D_OK;

@@ -39,19 +39,37 @@ __global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, 
 // memory, so the host need not see them before this pass); partial ||w||^2.
 // nrm != nullptr: scl_c = (1 / sqrt(nrm[c]))^2 from device memory, the square
 // of the slot scale the host applies to the Hessenberg entries.
+// dpart (short partial columns only): every block sums the dots partials
+// dpart[q*dnblk + b] itself -- one wave per column, lanes strided over the
+// blocks, a fixed order -- instead of reading d, and block 0 writes the sums
+// to hrow; no separate reduction launch.
 template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                     const double* __restrict__ d, CoefPack scl,
                                                     const double* __restrict__ nrm, cpx<R>* __restrict__ w,
-                                                    long n, double* __restrict__ part)
+                                                    long n, double* __restrict__ part,
+                                                    const double* __restrict__ dpart, int dnblk,
+                                                    double* __restrict__ hrow)
 {
+    __shared__ double dsum[2 * K];
+    if (dpart) {
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        for (int q = wid; q < 2 * K; q += KB / 64) {
+            double s = 0.0;
+            for (int b = lane; b < dnblk; b += 64) s += dpart[long(q) * dnblk + b];
+            s = kb::wave_sum(s);
+            if (lane == 0) dsum[q] = s;
+        }
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x < 2 * K) hrow[threadIdx.x] = dsum[threadIdx.x];
+    }
     double ar[K], ai[K];
 #pragma unroll
     for (int c = 0; c < K; ++c) {
         const double inv = nrm ? 1.0 / sqrt(nrm[c]) : 0.0;
         const double sc = nrm ? inv * inv : scl.v[c];
-        ar[c] = sc * d[2 * c];
-        ai[c] = sc * d[2 * c + 1];
+        ar[c] = sc * (dpart ? dsum[2 * c] : d[2 * c]);
+        ai[c] = sc * (dpart ? dsum[2 * c + 1] : d[2 * c + 1]);
     }
     kb::update_body<R, K, NT>(V, ldv, ar, ai, w, n, part, blockIdx.x, gridDim.x);
 }
@@ -176,7 +194,8 @@ hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n
 
 template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
-                         long n, double* part, int nblk, hipStream_t s, const double* nrm)
+                         long n, double* part, int nblk, hipStream_t s, const double* nrm, const double* dpart,
+                         double* hrow)
 {
     if (k < 1) return hipErrorInvalidValue;
     const bool nt = stream_basis<R>(k, n);
@@ -184,10 +203,10 @@ hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, cons
         constexpr int K = decltype(kc)::value;
         if (nt)
             hipLaunchKernelGGL((update_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, nrm, w, n,
-                               part);
+                               part, dpart, nblk, hrow);
         else
             hipLaunchKernelGGL((update_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, nrm, w, n,
-                               part);
+                               part, dpart, nblk, hrow);
         return hipGetLastError();
     });
 }
@@ -248,7 +267,8 @@ hipError_t launch_norm2(const cpx<R>* x, long n, double* part, int nblk, hipStre
     template hipError_t launch_dots<R>(const cpx<R>*, long, int, const cpx<R>*, long, double*, int,        \
                                        hipStream_t);                                                        \
     template hipError_t launch_update<R>(const cpx<R>*, long, int, const double*, const CoefPack&, cpx<R>*,  \
-                                         long, double*, int, hipStream_t, const double*);                   \
+                                         long, double*, int, hipStream_t, const double*, const double*,     \
+                                         double*);                                                          \
     template hipError_t launch_combine<R>(const cpx<R>*, long, int, const CoefPack&, cpx<R>*, long, int,    \
                                           hipStream_t);                                                     \
     template hipError_t launch_norm2<R>(const cpx<R>*, long, double*, int, hipStream_t);

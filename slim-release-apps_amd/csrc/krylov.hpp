@@ -21,9 +21,14 @@ hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n
 // a_c = scl.v[c] * (d[2c] + i d[2c+1]); d is device memory.  With nrm (device
 // memory, ||v_c||^2 per slot) the scale is (1 / sqrt(nrm[c]))^2 instead of
 // scl.v[c], so the pass needs nothing from the host.
+// dpart (optional, short columns): the dots partials (nblk per column, 2k
+// columns); each block sums them itself instead of reading d, and block 0
+// writes the 2k sums to hrow.
 template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
-                         long n, double* part, int nblk, hipStream_t s, const double* nrm = nullptr);
+                         long n, double* part, int nblk, hipStream_t s, const double* nrm = nullptr,
+                         const double* dpart = nullptr, double* hrow = nullptr);
+constexpr int kInBlockDots = 64;  // dots columns of at most this many partials are summed in the update pass
 template <typename R>
 hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, cpx<R>* x, long n, int nblk,
                           hipStream_t s);
