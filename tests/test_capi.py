@@ -98,3 +98,23 @@ def test_argument_errors_without_gpu():
                             P(z), P(z))
     assert code == HELM3D_ERR_XSIZE
 
+
+
+def test_product_path_has_no_oracle_or_fallback():
+    """The package and bench's timed legs never reach the CPU checker: no
+    module of the package names it, and a missing library raises."""
+    pkg = os.path.join(REPO, "slim-release-apps_amd")
+    for root, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h", ".c")):
+                text = open(os.path.join(root, f), errors="replace").read()
+                assert not re.search(r"\boracle\b|fgmres_oracle|helm27_oracle|_ref/", text), f
+    from helm3d import _lib
+    saved = (_lib.LIB_PATH, _lib._lib)
+    try:
+        _lib.LIB_PATH = os.path.join(REPO, "no-such-dir", "libhelm3d.so")
+        _lib._lib = None
+        with pytest.raises(RuntimeError, match="is missing"):
+            _lib.lib()
+    finally:
+        _lib.LIB_PATH, _lib._lib = saved
