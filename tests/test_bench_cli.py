@@ -1,0 +1,26 @@
+"""CPU checks of bench.py's command line (no GPU needed)."""
+import os
+import subprocess
+import sys
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _run(*args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), *args], capture_output=True,
+                          text=True, timeout=120, env=e)
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    p = _run("--gpus", "2", "--steps", "1", "--warmup", "0")
+    assert p.returncode == 2
+    assert "measures one GPU" in p.stderr
+    assert p.stdout == ""
+
+
+def test_world_size_above_one_exits_with_a_message():
+    p = _run("--steps", "1", env={"WORLD_SIZE": "4"})
+    assert p.returncode == 2
+    assert "WORLD_SIZE 4" in p.stderr
