@@ -9,29 +9,35 @@ h = 10 m, 6 Hz, velocity 1500->4000 m/s linear in depth with a seeded smooth
 +-10% perturbation, unit point source at (100,100,5).  Synthetic data, inputs
 resident in HBM before the timed region.
 
-value = solved RHS/s over the timed steps.  The JSON line also carries
+value = solved RHS/s over the timed steps (nothing instrumented inside them).
+The JSON line also carries
   roofline      the kernel class that takes most of the solve's time (the
                 classical Gram-Schmidt update w -= V h with ||w||^2,
                 16 (K+2) B/pt for K basis vectors): algorithmic bytes over
                 its launch durations, timed with HIP events on the library's
-                stream inside the timed solves; traffic from the rocprofv3
-                FETCH_SIZE / WRITE_SIZE summary in profiles/;
+                stream over a second pass of the same K solves (the rate of
+                that instrumented pass is reported beside value); traffic
+                from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+                profiles/;
   stencil_roofline  the same for the stencil (40 B/pt matvec, 56 B/pt
                 fused residual);
   mixed_batched the batched mixed-precision solve of the same model with 8
-                sources on a surface line (FP64 flexible GMRES(20), 20
-                complex64 BiCGStab iterations per outer step, all 8 right-hand
-                sides in one launch per kernel type): RHS/s, ratio to the
-                plain path, and each solution's error against the complex128
-                solution of the same source;
+                sources on a surface line (FP64 flexible GMRES, 20 complex64
+                BiCGStab iterations per outer step, all 8 right-hand sides in
+                one launch per kernel type): RHS/s, ratio to the plain path,
+                and each solution's error against the complex128 solution of
+                the same source;
+  multigrid, multigrid_sources, batched_uneven  the other solver forms;
   fwi_grads_per_min  least-squares FWI gradients w.r.t. squared slowness on
                 256x256x128 + 20-pt PML at 12.5 m (16 sources, 64x64
-                receivers, 4 and 6 Hz; batched mixed solves, tol 1e-6): one
-                full gradient (32 forward + 32 adjoint solves) timed;
+                receivers, 4 and 6 Hz, solves to 1e-6): one full gradient
+                (32 forward + 32 adjoint solves) timed;
   cpu_baseline  the reference's own C stencil (do_Hmvp compiled from the
-                reference header in place, OpenMP) driving a C restatement of
-                FGMRES.m on the host cores, timed on one restart cycle of the
-                same solve and scaled by the GPU's iteration count.
+                reference header in place, OpenMP, one thread per usable
+                host core, bound close to cores) driving a C restatement of
+                FGMRES.m: the whole 84^3 (64^3 + 10-pt PML) solve timed end to
+                end beside the same solve on the GPU, the 240x240x140 matvec,
+                and a bounded sample of the 240x240x140 solve.
 """
 from __future__ import annotations
 
@@ -48,8 +54,47 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "solved RHS/sec at 1e-6 FP64 residual + FWI grads/min; matvec % HBM peak"
 
 
-def _cpu_threads():
-    return max(1, min(16, os.cpu_count() or 1))
+def host_cpu_info():
+    """The host CPUs this process may use: the affinity mask, the cgroup CPU
+    quota (cpu.max) and the OpenMP share the environment sets
+    (OMP_NUM_THREADS); the CPU baseline uses all of them, as many threads as
+    the smallest of the three allows."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    omp_n = int(omp) if omp and omp.isdigit() and int(omp) > 0 else None
+    threads = affinity
+    if quota is not None:
+        threads = min(threads, max(1, int(quota)))
+    if omp_n is not None:
+        threads = min(threads, omp_n)
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota_cpus": quota, "OMP_NUM_THREADS": omp,
+            "threads": max(1, threads), "model": model}
+
+
+def _omp_binding():
+    """OpenMP thread placement of the CPU baseline (one thread per core,
+    packed): set before libgomp initialises in the oracle libraries."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
 
 def _alg_bytes(kind, iters, cycles, m, N, stencil_launches):
@@ -83,42 +128,80 @@ def _traffic_from_profile(name):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(cfg, wn, b, gpu_iters, sample_iters):
+def cpu_baseline(cfg2, wn2, gpu_c2_iters, sample_iters):
+    """The reference's C stencil (do_Hmvp compiled from its header in place,
+    gcc -O3 -fopenmp, else the oracle's C restatement) driving the C
+    restatement of FGMRES.m on the host cores:
+      * the whole C1 solve (84^3, 5 Hz, one impulse, FGMRES(20) to 1e-6),
+        timed end to end, beside the same solve on the GPU in this run;
+      * the C2 matvec (2 warm-ups, median of 10);
+      * a bounded sample of the C2 solve (sample_iters Arnoldi steps),
+        scaled by the GPU solve's iteration count, for reference."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import numpy as np
+    import torch
     import oracle as O
-    nt = cfg.nt
-    h3 = (cfg.h,) * 3
-    p6 = np.array(cfg.npml6)
-    th = _cpu_threads()
+    from helm3d import Helm3d
+    from helm3d.models import config_c1, extend, point_source, vec, wavenumber
+    info_cpu = host_cpu_info()
+    th = info_cpu["threads"]
     use_ref = O.ref_available()
+    # C1: CPU solve and GPU solve of the same system
+    c1 = config_c1()
+    wn1 = vec(wavenumber(extend(c1.v, c1.npml), c1.freq))
+    b1 = point_source(c1, c1.sources[0])
+    h1, p1 = (c1.h,) * 3, np.array(c1.npml6)
     t0 = time.perf_counter()
-    _, info = O.fgmres("forw", wn, h3, nt, p6, b, tol=1e-6, maxinnerit=20, nthreads=th, use_ref=use_ref,
-                       max_iters_cap=sample_iters)
-    dt = time.perf_counter() - t0
-    sec_per_iter = dt / max(1, info["iters"])
-    # matvec alone (2 warm-ups + median of 10)
-    x = np.random.default_rng(0).standard_normal(wn.size) + 0j
+    x1, i1 = O.fgmres("forw", wn1, h1, c1.nt, p1, b1, tol=1e-6, maxinnerit=20, nthreads=th, use_ref=use_ref)
+    cpu_s = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    H1 = Helm3d(c1.nt, h1, c1.npml6, torch.from_numpy(wn1).to(dev))
+    bt1 = torch.from_numpy(b1).to(dev)
+    H1.solve(bt1, tol=1e-6)  # warm-up (workspace)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        xg, ig = H1.solve(bt1, tol=1e-6)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / reps
+    del H1, xg
+    # C2 matvec
+    h2, p2 = (cfg2.h,) * 3, np.array(cfg2.npml6)
+    x = np.random.default_rng(0).standard_normal(wn2.size) + 0j
     for _ in range(2):
-        O.hmvp("forw", wn, h3, nt, p6, x, nthreads=th, use_ref=use_ref)
+        O.hmvp("forw", wn2, h2, cfg2.nt, p2, x, nthreads=th, use_ref=use_ref)
     ts = []
     for _ in range(10):
         t1 = time.perf_counter()
-        O.hmvp("forw", wn, h3, nt, p6, x, nthreads=th, use_ref=use_ref)
+        O.hmvp("forw", wn2, h2, cfg2.nt, p2, x, nthreads=th, use_ref=use_ref)
         ts.append(time.perf_counter() - t1)
     mv = float(np.median(ts))
+    # bounded sample of the C2 solve
+    b2 = point_source(cfg2, cfg2.sources[0])
+    t0 = time.perf_counter()
+    _, i2 = O.fgmres("forw", wn2, h2, cfg2.nt, p2, b2, tol=1e-6, maxinnerit=20, nthreads=th, use_ref=use_ref,
+                     max_iters_cap=sample_iters)
+    c2_s = time.perf_counter() - t0
+    kind = "reference" if use_ref else "port"
     return {
-        "value": 1.0 / (sec_per_iter * gpu_iters),
+        "value": 1.0 / cpu_s,
         "unit": "RHS/s",
         "cores": th,
-        "kind": "reference" if use_ref else "port",
-        "sample": (f"{info['iters']} Arnoldi steps (+{len(info['res']) + 1} residual matvec) of the same C2 solve "
-                   f"in {dt:.1f} s on {th} OpenMP threads: reference do_Hmvp "
-                   f"({'compiled in place, gcc -O3 -fopenmp' if use_ref else 'oracle restatement'}) driving a C "
-                   f"restatement of FGMRES.m; scaled by the GPU solve's {gpu_iters} iterations"),
-        "matvec_ms": mv * 1e3,
-        "matvec_gpts": wn.size / mv / 1e9,
-        "matvec_GBps": 40.0 * wn.size / mv / 1e9,
+        "kind": kind,
+        "sample": (f"the whole C1 solve (64^3 + 10-pt PML = 84^3, 5 Hz, one impulse at (32,32,5), FGMRES(20) to "
+                   f"1e-6): {i1['iters']} iterations in {cpu_s:.1f} s on {th} OpenMP threads (OMP_PROC_BIND=close, "
+                   f"OMP_PLACES=cores), final relres {i1['relres']:.2e}; "
+                   f"{'reference do_Hmvp compiled in place (gcc -O3 -fopenmp)' if use_ref else 'oracle stencil'} "
+                   "driving a C restatement of FGMRES.m"),
+        "host": info_cpu,
+        "c1": {"cpu_rhs_per_s": 1.0 / cpu_s, "cpu_seconds": cpu_s, "cpu_iters": i1["iters"],
+               "gpu_rhs_per_s": 1.0 / gpu_s, "gpu_seconds": gpu_s, "gpu_iters": ig.iters,
+               "gpu_over_cpu": cpu_s / gpu_s},
+        "c2_matvec": {"ms": mv * 1e3, "gpts": wn2.size / mv / 1e9, "GBps": 40.0 * wn2.size / mv / 1e9},
+        "c2_solve_estimate": {"rhs_per_s": 1.0 / (c2_s / max(1, i2["iters"]) * gpu_c2_iters),
+                              "sample": f"{i2['iters']} Arnoldi steps of the C2 solve in {c2_s:.1f} s, scaled by "
+                                        f"the GPU solve's {gpu_c2_iters} iterations"},
     }
 
 
@@ -442,6 +525,7 @@ def main():
                          "(profiling workload); copy: K device copies of 256 MiB (PMC calibration); "
                          "mixed / uneven / fwi / mg / mg-sources: one of the side legs alone (profiling workloads)")
     args = ap.parse_args()
+    _omp_binding()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU: the hot path is one solve per GPU and this build runs on a single "
@@ -493,7 +577,7 @@ def main():
         assert info.converged, info
     torch.cuda.synchronize()
 
-    H.profile(True)
+    # timed region (value): K solves, nothing instrumented
     infos = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -502,10 +586,20 @@ def main():
         infos.append(info)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    prof = H.profile_read()
-    H.profile(False)
     for info in infos:
         assert info.converged and info.relres < 1e-6, info
+    # the same K solves again with every launch bracketed by HIP events on
+    # the library's stream: kernel breakdown and rooflines
+    H.profile(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        xp, ip = H.solve(bt, tol=1e-6, maxinnerit=m)
+        assert ip.converged and ip.iters == infos[-1].iters, ip
+    torch.cuda.synchronize()
+    elapsed_prof = time.perf_counter() - t1
+    prof = H.profile_read()
+    H.profile(False)
 
     iters = infos[-1].iters
     cycles = infos[-1].cycles
@@ -572,6 +666,7 @@ def main():
         "cycles_per_solve": cycles,
         "final_relres": infos[-1].relres,
         "kernel_breakdown": breakdown,
+        "instrumented_rhs_per_s": args.steps / elapsed_prof,
         "fwi_grads_per_min": None,
     }
     del H
@@ -596,7 +691,9 @@ def main():
         out["fwi_grads_per_min"], out["fwi"] = fwi_leg()
         print(f"[bench] fwi leg done: {out['fwi_grads_per_min']:.3f} grads/min", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, wn, b, iters, args.cpu_sample_iters)
+        out["cpu_baseline"] = cpu_baseline(cfg, wn, iters, args.cpu_sample_iters)
+        print(f"[bench] cpu baseline done: C1 {out['cpu_baseline']['c1']['cpu_seconds']:.1f} s on "
+              f"{out['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 

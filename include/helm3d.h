@@ -265,13 +265,18 @@ int helm3d_kacz_sweep(helm3d_ctx *ctx, int mode, double omega, int nsweeps, void
 
 /* Drop-in of y = Helm3d_kacz_sweep{,_wnc,_adj}_mex(wn,h,n,npml,w,x,b)
  * (helm3d_operto27_kaczswp.m:2-12, Helm3dkaczswp_mex.c:123-197) on host
- * split-complex arrays, one right-hand side: variant HELM3D_FORW, HELM3D_ADJ,
- * or either with HELM3D_WNC (complex wn honoured); xi, bi may be NULL.
- * Errors: HELM3D_ERR_NSIZE, HELM3D_ERR_XSIZE, HELM3D_ERR_BSIZE as the MEX. */
+ * split-complex arrays: variant HELM3D_FORW, HELM3D_ADJ, or either with
+ * HELM3D_WNC (complex wn honoured); xi, bi may be NULL (zero imaginary part).
+ * x, b and y hold x_cols columns of x_rows values (column-major, as
+ * mxGetPr delivers an N x P matrix); column s of y is one double sweep of
+ * column s of x against column s of b (Helm3dkaczswp_mex.c:159,182-186).
+ * Errors: HELM3D_ERR_NSIZE, HELM3D_ERR_XSIZE (x rows), HELM3D_ERR_BSIZE
+ * (b rows, or b columns != x columns) as the MEX. */
 int helm3d_kaczswp(int variant, const double *wnr, const double *wni, const double *h,
                    const double *n, long long n_len, const double *npml, double omega,
-                   const double *xr, const double *xi, long long x_rows, const double *br,
-                   const double *bi, long long b_rows, double *yr, double *yi);
+                   const double *xr, const double *xi, long long x_rows, long long x_cols,
+                   const double *br, const double *bi, long long b_rows, long long b_cols,
+                   double *yr, double *yi);
 
 /* CGMN (CGMN.m) on the Kaczmarz system of linearsolve.m:52-58 with
  * omega = 1.5 and one double sweep: b' = SWP(0, b), A' y = y - SWP(y, 0), CG
@@ -284,6 +289,29 @@ int helm3d_solve_cgmn(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
  * (linearsolve.m:52-63, solver CRMN with the KACZSWP preconditioner): two
  * inner products and one double sweep per iteration; info as for CGMN. */
 int helm3d_solve_crmn(helm3d_ctx *ctx, int mode, const void *b_dev, void *x_dev,
+                      const helm3d_solve_opts *opts, helm3d_solve_info *info);
+
+/* ---- host-array solves (solve-level gateways) ------------------------------ */
+
+/* Load H(wn) into ctx from MATLAB-style double arrays: n (n_len entries, must
+ * be 3 positive integers, else HELM3D_ERR_NSIZE), h[3], npml[6] as
+ * helm3d_mvp takes them, and wn as split real / imaginary host arrays (wni
+ * NULL: real wn).  Synchronous.  The device-memory form is helm3d_set_model;
+ * this one lets a MEX gateway keep a context without HIP calls of its own. */
+int helm3d_set_model_host(helm3d_ctx *ctx, const double *n, long long n_len, const double *h,
+                          const double *npml, const double *wnr, const double *wni);
+
+/* x = H \ b (mode 1) or H^H \ b (otherwise) for b_cols right-hand sides held
+ * as split real / imaginary host arrays of b_rows x b_cols values
+ * (column-major; bi NULL: real b), from x = 0: linearsolve.m:20-48 with the
+ * FGMRES solver of LinSolveOpts (tol, maxinnerit, maxit from opts).  One
+ * column runs FGMRES.m (helm3d_solve); several columns run FGMRES_MT.m
+ * (FGMRES.m:35-37; helm3d_batch_solve, HELM3D_BATCH_PLAIN).  xr, xi receive
+ * b_rows x b_cols values; info (may be NULL) b_cols entries.  Errors:
+ * HELM3D_ERR_BSIZE when b_rows != prod(n) of the model, HELM3D_ERR_NOMODEL;
+ * HELM3D_ERR_NOTCONV when a column ended above tol (x is still written). */
+int helm3d_solve_host(helm3d_ctx *ctx, int mode, const double *br, const double *bi,
+                      long long b_rows, long long b_cols, double *xr, double *xi,
                       const helm3d_solve_opts *opts, helm3d_solve_info *info);
 
 /* ---- multigrid-preconditioned FGMRES ------------------------------------------ */
@@ -300,6 +328,11 @@ typedef struct helm3d_mg helm3d_mg;
 int helm3d_mg_create(int nlevels, helm3d_ctx *const *ops, helm3d_ctx *const *smoothers,
                      helm3d_mg **out);
 int helm3d_mg_destroy(helm3d_mg *mg);
+/* Issue the hierarchy's work (transfers, solves, every level context) on
+ * `stream` (a hipStream_t, NULL: the default stream) from now on; callers
+ * that allocate b and x on their own stream bind it before each call, as
+ * helm3d_set_stream does for one context.  No counterpart in the reference. */
+int helm3d_mg_set_stream(helm3d_mg *mg, void *stream);
 /* direction 0: out(level) += P in(level+1); 1: out(level+1) = P^T in(level) / 8 */
 int helm3d_mg_transfer(helm3d_mg *mg, int level, int direction, const void *in_dev, void *out_dev);
 /* FGMRES (FGMRES.m, flexible, opts->maxinnerit restart) on level 0

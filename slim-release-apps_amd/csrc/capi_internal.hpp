@@ -213,6 +213,54 @@ struct helm3d_ctx {
 
 namespace helm3d {
 
+// Operator descriptor of H(wn) for the kernels: grid, PML tables (tab + off[q]
+// for xlo, xhi, ylo, yhi, zlo, zhi), wavenumber (real or interleaved complex,
+// device), the h-dependent constants and the interior index ranges.
+template <typename R>
+void fill_opdesc(OpDesc<R>& op, const int n[3], const HConst& k, const cpx<R>* tab, const size_t off[6],
+                 const void* wn, bool wnc, const int ilo[3], const int ihi[3])
+{
+    op.nx = n[0];
+    op.ny = n[1];
+    op.nz = n[2];
+    op.xlo = tab + off[0];
+    op.xhi = tab + off[1];
+    op.ylo = tab + off[2];
+    op.yhi = tab + off[3];
+    op.zlo = tab + off[4];
+    op.zhi = tab + off[5];
+    if (wnc) {
+        op.wn = nullptr;
+        op.wnc = static_cast<const cpx<R>*>(wn);
+    } else {
+        op.wn = static_cast<const R*>(wn);
+        op.wnc = nullptr;
+    }
+    op.wn_const = R(0);
+    for (int a = 0; a < 3; ++a) op.c[a] = R(k.c[a]);
+    op.xy = R(k.xy);
+    op.xz = R(k.xz);
+    op.yz = R(k.yz);
+    op.w3a2 = R(k.w3a2);
+    op.wn_coef = R(k.wn_coef);
+    op.wm2 = R(kWM2);
+    op.wm3 = R(kWM3);
+    op.wm4 = R(kWM4);
+    // interior coefficients: xi_x = xi_y = 2, side values 1
+    op.fi[0] = R(-2 * k.c[0] - 2 * k.c[1]);
+    op.fi[1] = R(k.c[0] + 2 * k.xy);
+    op.fi[2] = R(k.c[1] + 2 * k.xy);
+    op.fi[3] = R(-2 * k.xy);
+    op.gi[0] = R(2 * k.xz + 2 * k.yz);
+    op.gi[1] = R(-k.xz + 2 * k.w3a2);
+    op.gi[2] = R(-k.yz + 2 * k.w3a2);
+    op.gi[3] = R(-2 * k.w3a2);
+    for (int a = 0; a < 3; ++a) {
+        op.int_lo[a] = ilo[a];
+        op.int_hi[a] = ihi[a];
+    }
+}
+
 inline int bind(helm3d_ctx* c)
 {
     if (!c) return fail(HELM3D_ERR_ARG, "null context");

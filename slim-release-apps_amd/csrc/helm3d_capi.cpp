@@ -720,28 +720,7 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
     for (size_t q = 0; q < len; ++q) t32[q] = cpx<float>{float(t64[q].re), float(t64[q].im)};
     // interior index range per axis: slots i..i+2 all exactly (1, 1)
     int ilo[3], ihi[3];
-    for (int a = 0; a < 3; ++a) {
-        auto one = [&](size_t q) { return t64[q].re == 1.0 && t64[q].im == 0.0; };
-        auto ok = [&](int i) {
-            for (int sl = i; sl <= i + 2; ++sl)
-                if (!one(off[2 * a] + sl) || !one(off[2 * a + 1] + sl)) return false;
-            return true;
-        };
-        ilo[a] = 1;
-        ihi[a] = 0;
-        int first = -1, last = -1;
-        for (int i = 0; i < n[a]; ++i)
-            if (ok(i)) {
-                if (first < 0) first = i;
-                last = i;
-            }
-        bool contiguous = first >= 0;
-        for (int i = first; contiguous && i <= last; ++i) contiguous = ok(i);
-        if (contiguous) {
-            ilo[a] = first;
-            ihi[a] = last;
-        }
-    }
+    for (int a = 0; a < 3; ++a) pml_interior(n[a], npml[2 * a], npml[2 * a + 1], &ilo[a], &ihi[a]);
     CHK(ensure_t(&c->tab64, &c->tab64_cap, len));
     CHK(ensure_t(&c->tab32, &c->tab32_cap, len));
     HIPCHK(hipMemcpyAsync(c->tab64, t64.data(), len * sizeof(cpx<double>), hipMemcpyHostToDevice, c->stream));
@@ -758,51 +737,8 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
     else
         HIPCHK(launch_cast_real(static_cast<const double*>(c->wn64), static_cast<float*>(c->wn32), N, c->stream));
 
-    auto fill = [&](auto& op, auto* tab, auto* wn) {
-        using R = std::remove_reference_t<decltype(op.wn_const)>;
-        op.nx = n[0];
-        op.ny = n[1];
-        op.nz = n[2];
-        op.xlo = tab + off[0];
-        op.xhi = tab + off[1];
-        op.ylo = tab + off[2];
-        op.yhi = tab + off[3];
-        op.zlo = tab + off[4];
-        op.zhi = tab + off[5];
-        if (c->wnc) {
-            op.wn = nullptr;
-            op.wnc = reinterpret_cast<const cpx<R>*>(wn);
-        } else {
-            op.wn = reinterpret_cast<const R*>(wn);
-            op.wnc = nullptr;
-        }
-        op.wn_const = R(0);
-        for (int a = 0; a < 3; ++a) op.c[a] = R(c->hc.c[a]);
-        op.xy = R(c->hc.xy);
-        op.xz = R(c->hc.xz);
-        op.yz = R(c->hc.yz);
-        op.w3a2 = R(c->hc.w3a2);
-        op.wn_coef = R(c->hc.wn_coef);
-        op.wm2 = R(kWM2);
-        op.wm3 = R(kWM3);
-        op.wm4 = R(kWM4);
-        // interior coefficients: xi_x = xi_y = 2, side values 1
-        const HConst& k = c->hc;
-        op.fi[0] = R(-2 * k.c[0] - 2 * k.c[1]);
-        op.fi[1] = R(k.c[0] + 2 * k.xy);
-        op.fi[2] = R(k.c[1] + 2 * k.xy);
-        op.fi[3] = R(-2 * k.xy);
-        op.gi[0] = R(2 * k.xz + 2 * k.yz);
-        op.gi[1] = R(-k.xz + 2 * k.w3a2);
-        op.gi[2] = R(-k.yz + 2 * k.w3a2);
-        op.gi[3] = R(-2 * k.w3a2);
-        for (int a = 0; a < 3; ++a) {
-            op.int_lo[a] = ilo[a];
-            op.int_hi[a] = ihi[a];
-        }
-    };
-    fill(c->op64, c->tab64, c->wn64);
-    fill(c->op32, c->tab32, c->wn32);
+    fill_opdesc(c->op64, n, c->hc, c->tab64, off, c->wn64, c->wnc, ilo, ihi);
+    fill_opdesc(c->op32, n, c->hc, c->tab32, off, c->wn32, c->wnc, ilo, ihi);
     if (!same_grid) c->grid = stencil_grid<double>(n[0], n[1], n[2], 0);
     CHK(ensure_work(c));
     HIPCHK(hipStreamSynchronize(c->stream));

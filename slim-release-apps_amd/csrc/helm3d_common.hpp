@@ -83,6 +83,12 @@ struct PmlAxis {
     std::vector<std::complex<double>> lo, hi;
 };
 PmlAxis make_pml_axis(int n, int nlo, int nhi);
+// Slot t of both tables of one axis (no allocation).
+void pml_slot(int n, int nlo, int nhi, int t, std::complex<double>* lo, std::complex<double>* hi);
+// Interior index range of one axis: indices i whose slots i, i+1, i+2 all
+// hold xi_lo = xi_hi = 1 exactly, as one contiguous run [ilo, ihi] (ilo > ihi
+// when there is none).  No allocation.
+void pml_interior(int n, int nlo, int nhi, int* ilo, int* ihi);
 
 // Device-side description of one operator H(wn) on an nx*ny*nz grid
 // (x fastest, idx = i + j*nx + k*nx*ny as Helm3d_27pt.h:21-23).

@@ -82,7 +82,8 @@ int helm3d_kacz_sweep(helm3d_ctx* c, int mode, double omega, int nsweeps, void* 
 
 int helm3d_kaczswp(int variant, const double* wnr, const double* wni, const double* h, const double* n,
                    long long n_len, const double* npml, double omega, const double* xr, const double* xi,
-                   long long x_rows, const double* br, const double* bi, long long b_rows, double* yr, double* yi)
+                   long long x_rows, long long x_cols, const double* br, const double* bi, long long b_rows,
+                   long long b_cols, double* yr, double* yi)
 {
     if (n_len != 3 || !n) return fail(HELM3D_ERR_NSIZE, "n must be a 3-length vector");
     int ni[3];
@@ -94,16 +95,22 @@ int helm3d_kaczswp(int variant, const double* wnr, const double* wni, const doub
     const long long N = (long long)ni[0] * ni[1] * ni[2];
     if (x_rows != N) return fail(HELM3D_ERR_XSIZE, "x must have nx*ny*nz elements");
     if (b_rows != N) return fail(HELM3D_ERR_BSIZE, "b must have nx*ny*nz elements");
+    if (x_cols < 0) return fail(HELM3D_ERR_XSIZE, "x must have a non-negative number of columns");
+    // column s of y sweeps column s of x against column s of b
+    // (Helm3dkaczswp_mex.c:159,182-186)
+    if (b_cols != x_cols) return fail(HELM3D_ERR_BSIZE, "b must have as many columns as x");
     if (variant != HELM3D_FORW && variant != HELM3D_ADJ && variant != (HELM3D_FORW | HELM3D_WNC) &&
         variant != (HELM3D_ADJ | HELM3D_WNC))
         return fail(HELM3D_ERR_ARG, "bad variant");
     const bool wnc = variant & HELM3D_WNC;
-    if (!wnr || !h || !npml || !xr || !br || !yr || !yi || (wnc && !wni)) return fail(HELM3D_ERR_ARG, "null argument");
+    if (!wnr || !h || !npml || (wnc && !wni)) return fail(HELM3D_ERR_ARG, "null argument");
+    if (x_cols > 0 && (!xr || !br || !yr || !yi)) return fail(HELM3D_ERR_ARG, "null argument");
     int np[6];
     for (int a = 0; a < 6; ++a) {
         if (!(npml[a] >= 0)) return fail(HELM3D_ERR_ARG, "npml must be non-negative");
         np[a] = int(npml[a]);
     }
+    if (x_cols == 0) return HELM3D_OK;
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     helm3d_ctx* c = nullptr;
@@ -139,13 +146,16 @@ int helm3d_kaczswp(int variant, const double* wnr, const double* wni, const doub
         HIPCHK(hipMemcpyAsync(d_wn, wnr, size_t(N) * sizeof(double), hipMemcpyHostToDevice, st));
         CHK(helm3d_set_model(c, ni, h, np, d_wn, 0));
     }
-    CHK(up(xr, xi, d_x));
-    CHK(up(br, bi, d_b));
-    CHK(sweep(c, (variant & HELM3D_ADJ) ? 2 : 1, omega, d_x, d_b, 1));
-    HIPCHK(launch_deinterleave(d_x, d_pl, d_pl + N, N, st));
-    HIPCHK(hipMemcpyAsync(yr, d_pl, size_t(N) * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(yi, d_pl + N, size_t(N) * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    for (long long q = 0; q < x_cols; ++q) {
+        const size_t o = size_t(q) * size_t(N);
+        CHK(up(xr + o, xi ? xi + o : nullptr, d_x));
+        CHK(up(br + o, bi ? bi + o : nullptr, d_b));
+        CHK(sweep(c, (variant & HELM3D_ADJ) ? 2 : 1, omega, d_x, d_b, 1));
+        HIPCHK(launch_deinterleave(d_x, d_pl, d_pl + N, N, st));
+        HIPCHK(hipMemcpyAsync(yr + o, d_pl, size_t(N) * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(yi + o, d_pl + N, size_t(N) * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));  // the staging buffers serve the next column
+    }
     return HELM3D_OK;
 }
 

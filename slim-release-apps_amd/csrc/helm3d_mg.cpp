@@ -371,6 +371,17 @@ int helm3d_mg_destroy(helm3d_mg* mg)
     return HELM3D_OK;
 }
 
+int helm3d_mg_set_stream(helm3d_mg* mg, void* stream)
+{
+    if (!mg) return fail(HELM3D_ERR_ARG, "null hierarchy");
+    mg->stream = static_cast<hipStream_t>(stream);
+    for (auto& L : mg->lev) {
+        L.A->stream = mg->stream;
+        L.S->stream = mg->stream;
+    }
+    return HELM3D_OK;
+}
+
 int helm3d_mg_transfer(helm3d_mg* mg, int level, int direction, const void* in, void* out)
 {
     if (!mg || level < 0 || level + 1 >= mg->L || !in || !out) return fail(HELM3D_ERR_ARG, "bad argument");

@@ -84,17 +84,56 @@ std::complex<double> xi_upper(double x, double n, double nlo, double nhi)
 
 }  // namespace
 
+void pml_slot(int n, int nlo, int nhi, int t, std::complex<double>* lo, std::complex<double>* hi)
+{
+    // slot t holds 0-based index t-1, whose 1-based position is t
+    *lo = xi_lower(double(t), double(n), double(nlo), double(nhi));
+    *hi = xi_upper(double(t), double(n), double(nlo), double(nhi));
+}
+
 PmlAxis make_pml_axis(int n, int nlo, int nhi)
 {
     PmlAxis a;
     a.lo.resize(n + 2);
     a.hi.resize(n + 2);
-    for (int t = 0; t < n + 2; ++t) {
-        // slot t holds 0-based index t-1, whose 1-based position is t
-        a.lo[t] = xi_lower(double(t), double(n), double(nlo), double(nhi));
-        a.hi[t] = xi_upper(double(t), double(n), double(nlo), double(nhi));
-    }
+    for (int t = 0; t < n + 2; ++t) pml_slot(n, nlo, nhi, t, &a.lo[t], &a.hi[t]);
     return a;
+}
+
+void pml_interior(int n, int nlo, int nhi, int* ilo, int* ihi)
+{
+    auto one = [&](int t) {
+        std::complex<double> lo, hi;
+        pml_slot(n, nlo, nhi, t, &lo, &hi);
+        return lo == std::complex<double>(1.0, 0.0) && hi == std::complex<double>(1.0, 0.0);
+    };
+    // i qualifies when slots i, i+1, i+2 all hold exactly (1, 1); the
+    // qualifying indices must form one contiguous run, else none qualify
+    *ilo = 1;
+    *ihi = 0;
+    int first = -1, last = -1, run = 0;
+    bool prev_ok = false, contiguous = true;
+    for (int i = 0; i < n; ++i) {
+        bool ok;
+        if (i == 0) {
+            run = int(one(0)) + int(one(1)) + int(one(2));
+            ok = run == 3;
+        } else {
+            // slide the window i-1..i+1 -> i..i+2
+            run += int(one(i + 2)) - int(one(i - 1));
+            ok = run == 3;
+        }
+        if (ok) {
+            if (first < 0) first = i;
+            else if (!prev_ok) contiguous = false;
+            last = i;
+        }
+        prev_ok = ok;
+    }
+    if (first >= 0 && contiguous) {
+        *ilo = first;
+        *ihi = last;
+    }
 }
 
 }  // namespace helm3d

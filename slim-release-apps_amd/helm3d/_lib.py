@@ -122,7 +122,11 @@ _SIGS = [
     ("helm3d_batch_item_bytes", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
     ("helm3d_kacz_sweep", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _vp, _vp]),
     ("helm3d_kaczswp", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_longlong, _dp, ctypes.c_double,
-                                      _dp, _dp, ctypes.c_longlong, _dp, _dp, ctypes.c_longlong, _dp, _dp]),
+                                      _dp, _dp, ctypes.c_longlong, ctypes.c_longlong, _dp, _dp, ctypes.c_longlong,
+                                      ctypes.c_longlong, _dp, _dp]),
+    ("helm3d_set_model_host", ctypes.c_int, [_vp, _dp, ctypes.c_longlong, _dp, _dp, _dp, _dp]),
+    ("helm3d_solve_host", ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_longlong, ctypes.c_longlong, _dp,
+                                         _dp, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveInfo)]),
     ("helm3d_solve_cgmn", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
                                          ctypes.POINTER(SolveInfo)]),
     ("helm3d_solve_crmn", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
@@ -130,6 +134,7 @@ _SIGS = [
     ("helm3d_mg_create", ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                         ctypes.POINTER(_vp)]),
     ("helm3d_mg_destroy", ctypes.c_int, [_vp]),
+    ("helm3d_mg_set_stream", ctypes.c_int, [_vp, _vp]),
     ("helm3d_mg_transfer", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     ("helm3d_solve_mg", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts), _ip,
                                        ctypes.POINTER(SolveInfo)]),

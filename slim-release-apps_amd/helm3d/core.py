@@ -472,28 +472,33 @@ _SWEEP_VARIANTS = {"forw": _lib.HELM3D_FORW, "adj": _lib.HELM3D_ADJ,
 
 def helm3d_kaczswp_host(variant, wn, h, n, npml, omega, x, b) -> np.ndarray:
     """y = Helm3d_kacz_sweep{,_wnc,_adj}_mex(wn, h, n, npml, w, x, b) on host
-    arrays, one right-hand side (27-colour double sweep on the GPU)."""
+    arrays (27-colour double sweep on the GPU).  x and b are N-vectors or
+    N x P matrices; column s of y sweeps column s of x against column s of b
+    (Helm3dkaczswp_mex.c:159,182-186).  Returns an N-vector for vector x,
+    else N x P."""
     v = _SWEEP_VARIANTS[variant] if isinstance(variant, str) else int(variant)
     wn = np.asarray(wn).reshape(-1)
     n_arr = np.ascontiguousarray(np.asarray(n, dtype=np.float64).reshape(-1))
     x = np.asarray(x)
     b = np.asarray(b)
-    x_rows = x.shape[0] if x.ndim >= 1 else 1
-    b_rows = b.shape[0] if b.ndim >= 1 else 1
-    x = x.reshape(x_rows, -1)[:, 0] if x.ndim > 1 else x.reshape(-1)
-    b = b.reshape(b_rows, -1)[:, 0] if b.ndim > 1 else b.reshape(-1)
-    N = x.size
-    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
-    wnr = f(wn.real)
-    wni = f(wn.imag) if np.iscomplexobj(wn) else None
-    xr, xi = f(x.real), (f(x.imag) if np.iscomplexobj(x) else None)
-    br, bi = f(b.real), (f(b.imag) if np.iscomplexobj(b) else None)
-    hh = f(np.asarray(h).reshape(3))
-    pp = f(np.asarray(npml_flat(npml)))
-    yr = np.zeros(N)
-    yi = np.zeros(N)
+    vec_out = x.ndim <= 1
+    x2 = x.reshape(-1, 1) if x.ndim <= 1 else x.reshape(x.shape[0], -1)
+    b2 = b.reshape(-1, 1) if b.ndim <= 1 else b.reshape(b.shape[0], -1)
+    x_rows, x_cols = x2.shape
+    b_rows, b_cols = b2.shape
+    # column-major N x P blocks, as mxGetPr delivers them
+    f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).T).reshape(-1)  # noqa: E731
+    wnr = np.ascontiguousarray(wn.real, dtype=np.float64)
+    wni = np.ascontiguousarray(wn.imag, dtype=np.float64) if np.iscomplexobj(wn) else None
+    xr, xi = f(x2.real), (f(x2.imag) if np.iscomplexobj(x2) else None)
+    br, bi = f(b2.real), (f(b2.imag) if np.iscomplexobj(b2) else None)
+    hh = np.ascontiguousarray(np.asarray(h, dtype=np.float64).reshape(3))
+    pp = np.ascontiguousarray(np.asarray(npml_flat(npml), dtype=np.float64))
+    yr = np.zeros(x_rows * x_cols)
+    yi = np.zeros(x_rows * x_cols)
     P = lambda a: a.ctypes.data_as(_lib._dp) if a is not None else None  # noqa: E731
     code = lib().helm3d_kaczswp(v, P(wnr), P(wni), P(hh), P(n_arr), n_arr.size, P(pp), float(omega), P(xr), P(xi),
-                                x_rows, P(br), P(bi), b_rows, P(yr), P(yi))
+                                x_rows, x_cols, P(br), P(bi), b_rows, b_cols, P(yr), P(yi))
     check(code, ids=_lib.MEX_SWEEP_ERR_IDS)
-    return yr + 1j * yi
+    y = (yr + 1j * yi).reshape(x_cols, x_rows).T
+    return y[:, 0] if vec_out else y

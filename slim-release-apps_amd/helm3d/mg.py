@@ -117,6 +117,11 @@ class MLGMRES:
         except Exception:
             pass
 
+    def _bind(self):
+        """Run the hierarchy on the caller's current stream (where b, x and
+        the outputs were allocated or zeroed)."""
+        check(lib().helm3d_mg_set_stream(self._mg, ctypes.c_void_p(_stream_ptr())))
+
     def transfer(self, level: int, x: torch.Tensor, direction: str) -> torch.Tensor:
         """"prolong": P x (level+1 -> level); "restrict": P^T x / 8 (level -> level+1)."""
         if direction == "prolong":
@@ -127,6 +132,7 @@ class MLGMRES:
             d = 1
         x = x.to(device=self.device, dtype=torch.complex128).contiguous()
         with torch.cuda.device(self.device):
+            self._bind()
             check(lib().helm3d_mg_transfer(self._mg, int(level), d, ctypes.c_void_p(x.data_ptr()),
                                            ctypes.c_void_p(out.data_ptr())))
         return out
@@ -145,6 +151,7 @@ class MLGMRES:
         info = _lib.SolveInfo()
         kk = (ctypes.c_int * 4)(*self.k)
         with torch.cuda.device(self.device):
+            self._bind()
             fn = lib().helm3d_solve_mg if precision == "c128" else lib().helm3d_solve_mg_mixed
             code = fn(self._mg, int(mode), ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
                       ctypes.byref(o), kk, ctypes.byref(info))
@@ -166,7 +173,9 @@ class MGPool:
         self.device = device
         self.streams = [torch.cuda.Stream(device) for _ in range(max(1, int(workers)))]
         self.hier = []
+        main = torch.cuda.current_stream(device)
         for s in self.streams:
+            s.wait_stream(main)  # v may still be in flight on the caller's stream
             with torch.cuda.device(device), torch.cuda.stream(s):
                 self.hier.append(MLGMRES(v, n_phys, h, npml, freq, unit, nlevels, k, device, free_surface,
                                          autotune))
@@ -182,7 +191,9 @@ class MGPool:
         return self.hier[0].grids
 
     def set_model(self, v, freq: float):
+        main = torch.cuda.current_stream(self.device)
         for s, M in zip(self.streams, self.hier):
+            s.wait_stream(main)  # v may still be in flight on the caller's stream
             with torch.cuda.device(self.device), torch.cuda.stream(s):
                 M.set_model(v, freq)
         for s in self.streams:

@@ -91,12 +91,23 @@ def test_argument_errors_without_gpu():
     assert L.helm3d_mvp(0, P(z), None, P(h), P(n3), 3, P(bad), P(z), P(z), P(z), None, 8, 1) == HELM3D_ERR_ARG
     # complex-wn variant without the imaginary part
     assert L.helm3d_mvp(4, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), P(z), P(z), P(z), None, 8, 1) == HELM3D_ERR_ARG
-    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 8, P(z), None, 7,
+    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 8, 1, P(z), None, 7, 1,
                             P(z), P(z))
     assert code == HELM3D_ERR_BSIZE
-    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 6, P(z), None, 8,
+    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 6, 1, P(z), None, 8, 1,
                             P(z), P(z))
     assert code == HELM3D_ERR_XSIZE
+    # b must have as many columns as x (column s sweeps x(:,s) against b(:,s))
+    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, P(z), None, 8, 1, P(z), None, 8, 2,
+                            P(z), P(z))
+    assert code == HELM3D_ERR_BSIZE
+    # zero columns: nothing to sweep, no device needed
+    code = L.helm3d_kaczswp(0, P(z), None, P(h), P(n3), 3, P(np.zeros(6)), 1.5, None, None, 8, 0, None, None, 8, 0,
+                            None, None)
+    assert code == 0
+    # host solves: a null context is an argument error
+    assert L.helm3d_solve_host(None, 1, P(z), None, 8, 1, P(z), P(z), None, None) == HELM3D_ERR_ARG
+    assert L.helm3d_set_model_host(None, P(n3), 3, P(h), P(np.zeros(6)), P(z), None) == HELM3D_ERR_ARG
 
 
 
