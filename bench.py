@@ -90,11 +90,16 @@ def host_cpu_info():
             "threads": max(1, threads), "model": model}
 
 
-def _omp_binding():
-    """OpenMP thread placement of the CPU baseline (one thread per core,
-    packed): set before libgomp initialises in the oracle libraries."""
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+def _cpu_env():
+    """Environment of the CPU-baseline child process: OpenMP threads bound one
+    per core, packed.  Set only there: bound at start-up, an OpenMP runtime
+    pins the initial thread to one core, and every thread it creates later
+    (the GPU legs' host threads, the affinity query) would inherit that one
+    core."""
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "close"
+    env["OMP_PLACES"] = "cores"
+    return env
 
 
 def _alg_bytes(kind, iters, cycles, m, N, stencil_launches):
@@ -128,35 +133,82 @@ def _traffic_from_profile(name):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(cfg2, wn2, gpu_c2_iters, sample_iters):
-    """The reference's C stencil (do_Hmvp compiled from its header in place,
-    gcc -O3 -fopenmp, else the oracle's C restatement) driving the C
-    restatement of FGMRES.m on the host cores:
-      * the whole C1 solve (84^3, 5 Hz, one impulse, FGMRES(20) to 1e-6),
-        timed end to end, beside the same solve on the GPU in this run;
+def _load_models():
+    """helm3d/models.py (numpy / scipy only) without importing the package,
+    which would import torch and its OpenMP runtime."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("_bench_models", os.path.join(PKG, "helm3d", "models.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[spec.name] = mod  # dataclasses look their module up while the file runs
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_mode(args):
+    """CPU part of the baseline (run as a child process with _cpu_env()): the
+    reference's C stencil (do_Hmvp compiled from its header in place, gcc -O3
+    -fopenmp, else the oracle's C restatement) driving the C restatement of
+    FGMRES.m on the host cores:
+      * the whole C1 solve (84^3, 5 Hz, one impulse, FGMRES(20) to 1e-6);
       * the C2 matvec (2 warm-ups, median of 10);
-      * a bounded sample of the C2 solve (sample_iters Arnoldi steps),
-        scaled by the GPU solve's iteration count, for reference."""
+      * a bounded sample of the C2 solve (sample_iters Arnoldi steps).
+    Prints one JSON line; touches no GPU."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import numpy as np
-    import torch
     import oracle as O
+    M = _load_models()
+    host = host_cpu_info()
+    th = host["threads"]
+    use_ref = O.ref_available()
+    c1 = M.config_c1()
+    wn1 = M.vec(M.wavenumber(M.extend(c1.v, c1.npml), c1.freq))
+    b1 = M.point_source(c1, c1.sources[0])
+    t0 = time.perf_counter()
+    _, i1 = O.fgmres("forw", wn1, (c1.h,) * 3, c1.nt, np.array(c1.npml6), b1, tol=1e-6, maxinnerit=20, nthreads=th,
+                     use_ref=use_ref)
+    c1_s = time.perf_counter() - t0
+    c2 = M.config_c2()
+    wn2 = M.vec(M.wavenumber(M.extend(c2.v, c2.npml), c2.freq))
+    h2, p2 = (c2.h,) * 3, np.array(c2.npml6)
+    x = np.random.default_rng(0).standard_normal(wn2.size) + 0j
+    for _ in range(2):
+        O.hmvp("forw", wn2, h2, c2.nt, p2, x, nthreads=th, use_ref=use_ref)
+    ts = []
+    for _ in range(10):
+        t1 = time.perf_counter()
+        O.hmvp("forw", wn2, h2, c2.nt, p2, x, nthreads=th, use_ref=use_ref)
+        ts.append(time.perf_counter() - t1)
+    mv = float(np.median(ts))
+    b2 = M.point_source(c2, c2.sources[0])
+    t0 = time.perf_counter()
+    _, i2 = O.fgmres("forw", wn2, h2, c2.nt, p2, b2, tol=1e-6, maxinnerit=20, nthreads=th, use_ref=use_ref,
+                     max_iters_cap=args.cpu_sample_iters)
+    c2_s = time.perf_counter() - t0
+    print(json.dumps({"mode": "cpu-baseline", "host": host, "threads": th, "use_ref": use_ref,
+                      "omp": {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES", "OMP_NUM_THREADS")},
+                      "c1_seconds": c1_s, "c1_iters": i1["iters"], "c1_relres": i1["relres"],
+                      "c2_matvec_s": mv, "c2_points": int(wn2.size),
+                      "c2_sample_seconds": c2_s, "c2_sample_iters": i2["iters"]}))
+
+
+def cpu_baseline(gpu_c2_iters, sample_iters):
+    """The CPU baseline beside the GPU: the CPU part in a child process
+    (cpu_mode), the same C1 solve on the GPU here."""
+    import subprocess
+    import numpy as np
+    import torch
     from helm3d import Helm3d
     from helm3d.models import config_c1, extend, point_source, vec, wavenumber
-    info_cpu = host_cpu_info()
-    th = info_cpu["threads"]
-    use_ref = O.ref_available()
-    # C1: CPU solve and GPU solve of the same system
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--mode", "cpu-baseline", "--cpu-sample-iters",
+                        str(sample_iters)], env=_cpu_env(), capture_output=True, text=True, timeout=1200)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline child failed ({r.returncode}): {r.stderr[-2000:]}")
+    cpu = json.loads(r.stdout.strip().splitlines()[-1])
     c1 = config_c1()
     wn1 = vec(wavenumber(extend(c1.v, c1.npml), c1.freq))
-    b1 = point_source(c1, c1.sources[0])
-    h1, p1 = (c1.h,) * 3, np.array(c1.npml6)
-    t0 = time.perf_counter()
-    x1, i1 = O.fgmres("forw", wn1, h1, c1.nt, p1, b1, tol=1e-6, maxinnerit=20, nthreads=th, use_ref=use_ref)
-    cpu_s = time.perf_counter() - t0
     dev = torch.device("cuda:0")
-    H1 = Helm3d(c1.nt, h1, c1.npml6, torch.from_numpy(wn1).to(dev))
-    bt1 = torch.from_numpy(b1).to(dev)
+    H1 = Helm3d(c1.nt, (c1.h,) * 3, c1.npml6, torch.from_numpy(wn1).to(dev))
+    bt1 = torch.from_numpy(point_source(c1, c1.sources[0])).to(dev)
     H1.solve(bt1, tol=1e-6)  # warm-up (workspace)
     torch.cuda.synchronize()
     reps = 5
@@ -166,42 +218,29 @@ def cpu_baseline(cfg2, wn2, gpu_c2_iters, sample_iters):
     torch.cuda.synchronize()
     gpu_s = (time.perf_counter() - t0) / reps
     del H1, xg
-    # C2 matvec
-    h2, p2 = (cfg2.h,) * 3, np.array(cfg2.npml6)
-    x = np.random.default_rng(0).standard_normal(wn2.size) + 0j
-    for _ in range(2):
-        O.hmvp("forw", wn2, h2, cfg2.nt, p2, x, nthreads=th, use_ref=use_ref)
-    ts = []
-    for _ in range(10):
-        t1 = time.perf_counter()
-        O.hmvp("forw", wn2, h2, cfg2.nt, p2, x, nthreads=th, use_ref=use_ref)
-        ts.append(time.perf_counter() - t1)
-    mv = float(np.median(ts))
-    # bounded sample of the C2 solve
-    b2 = point_source(cfg2, cfg2.sources[0])
-    t0 = time.perf_counter()
-    _, i2 = O.fgmres("forw", wn2, h2, cfg2.nt, p2, b2, tol=1e-6, maxinnerit=20, nthreads=th, use_ref=use_ref,
-                     max_iters_cap=sample_iters)
-    c2_s = time.perf_counter() - t0
-    kind = "reference" if use_ref else "port"
+    th, c1_s, N2 = cpu["threads"], cpu["c1_seconds"], cpu["c2_points"]
     return {
-        "value": 1.0 / cpu_s,
+        "value": 1.0 / c1_s,
         "unit": "RHS/s",
         "cores": th,
-        "kind": kind,
+        "kind": "reference" if cpu["use_ref"] else "port",
         "sample": (f"the whole C1 solve (64^3 + 10-pt PML = 84^3, 5 Hz, one impulse at (32,32,5), FGMRES(20) to "
-                   f"1e-6): {i1['iters']} iterations in {cpu_s:.1f} s on {th} OpenMP threads (OMP_PROC_BIND=close, "
-                   f"OMP_PLACES=cores), final relres {i1['relres']:.2e}; "
-                   f"{'reference do_Hmvp compiled in place (gcc -O3 -fopenmp)' if use_ref else 'oracle stencil'} "
-                   "driving a C restatement of FGMRES.m"),
-        "host": info_cpu,
-        "c1": {"cpu_rhs_per_s": 1.0 / cpu_s, "cpu_seconds": cpu_s, "cpu_iters": i1["iters"],
+                   f"1e-6): {cpu['c1_iters']} iterations in {c1_s:.1f} s on {th} OpenMP threads (OMP_PROC_BIND=close, "
+                   f"OMP_PLACES=cores), final relres {cpu['c1_relres']:.2e}; "
+                   f"{'reference do_Hmvp compiled in place (gcc -O3 -fopenmp)' if cpu['use_ref'] else 'oracle stencil'}"
+                   " driving a C restatement of FGMRES.m"),
+        "host": cpu["host"],
+        "omp": cpu["omp"],
+        "c1": {"cpu_rhs_per_s": 1.0 / c1_s, "cpu_seconds": c1_s, "cpu_iters": cpu["c1_iters"],
                "gpu_rhs_per_s": 1.0 / gpu_s, "gpu_seconds": gpu_s, "gpu_iters": ig.iters,
-               "gpu_over_cpu": cpu_s / gpu_s},
-        "c2_matvec": {"ms": mv * 1e3, "gpts": wn2.size / mv / 1e9, "GBps": 40.0 * wn2.size / mv / 1e9},
-        "c2_solve_estimate": {"rhs_per_s": 1.0 / (c2_s / max(1, i2["iters"]) * gpu_c2_iters),
-                              "sample": f"{i2['iters']} Arnoldi steps of the C2 solve in {c2_s:.1f} s, scaled by "
-                                        f"the GPU solve's {gpu_c2_iters} iterations"},
+               "gpu_over_cpu": c1_s / gpu_s},
+        "c2_matvec": {"ms": cpu["c2_matvec_s"] * 1e3, "gpts": N2 / cpu["c2_matvec_s"] / 1e9,
+                      "GBps": 40.0 * N2 / cpu["c2_matvec_s"] / 1e9},
+        "c2_solve_estimate": {"rhs_per_s": 1.0 / (cpu["c2_sample_seconds"] / max(1, cpu["c2_sample_iters"]) *
+                                                  gpu_c2_iters),
+                              "sample": f"{cpu['c2_sample_iters']} Arnoldi steps of the C2 solve in "
+                                        f"{cpu['c2_sample_seconds']:.1f} s, scaled by the GPU solve's "
+                                        f"{gpu_c2_iters} iterations"},
     }
 
 
@@ -395,7 +434,7 @@ def mg_sources_leg(plain_rhs_per_s, workers=4):
             "dtype": "c64 V-cycle / f64 outer"}
 
 
-def fwi_leg(solver="mg"):
+def fwi_leg(solver="mg", workers=8):
     """One least-squares FWI gradient on the 256x256x128 workload; every
     solve to 1e-6 with FGMRES preconditioned by a 4-level complex64 V-cycle,
     eight solves at a time on eight streams (solver "mg"), or with batched
@@ -407,10 +446,10 @@ def fwi_leg(solver="mg"):
     dev = torch.device("cuda:0")
     if solver == "mg":
         # 8 hierarchies: the 16 sources of a frequency split evenly
-        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed", mg_workers=8,
-                       mg_k=MG_K_TUNED)
-        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, 8 solves at a "
-               "time on 8 streams")
+        opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mg", mg_levels=4, mg_precision="mixed",
+                       mg_workers=workers, mg_k=MG_K_TUNED)
+        how = (f"FGMRES(20) + 4-level complex64 ML_GMRES V-cycle (k = {MG_K_TUNED}), FP64 outer, {workers} "
+               f"solve{'s' if workers > 1 else ''} at a time on {workers} stream{'s' if workers > 1 else ''}")
     else:
         opts = FWIOpts(pml=20, pml_max=20, tol=1e-6, solver="mixed", batch=16, inner_iters=20)
         how = "batched mixed-precision solves (16 sources per batch)"
@@ -476,8 +515,7 @@ def matvec_mode(args, cfg, wn):
     dev = torch.device("cuda:0")
     H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
     if args.launch:
-        kc, ty, pf = (int(a) for a in args.launch.split(","))
-        H.set_launch(kc, ty, pf)
+        H.set_launch(*(int(a) for a in args.launch.split(",")))
     elif not args.no_tune:
         H.autotune(mode=1, precision="c128")
     x = torch.from_numpy(np.random.default_rng(0).standard_normal(H.N) + 0j).to(dev)
@@ -512,20 +550,24 @@ def main():
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--launch", default=None,
-                    help="--mode matvec only: stencil launch shape 'kchunk,tile_rows,prefetch' (no autotuning)")
+                    help="--mode matvec only: stencil launch shape 'kchunk,tile_rows,prefetch[,rows_per_thread]' "
+                         "(no autotuning)")
     ap.add_argument("--cpu-sample-iters", type=int, default=20)
+    ap.add_argument("--fwi-workers", type=int, default=8,
+                    help="--mode fwi only: multigrid hierarchies solving at once (host threads)")
     ap.add_argument("--mixed-restart", type=int, default=None,
                     help="outer FGMRES restart of the batched mixed-precision legs (default MIXED_RESTART)")
     ap.add_argument("--mg-config", default=None,
                     help="--mode mg only: one 'levels,precision[,k1,k2,k3,k4]' set (e.g. 4,mixed or "
                          "4,mixed,1,5,3,5) instead of the default list")
-    ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg", "mg-sources"),
+    ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg", "mg-sources",
+                                       "cpu-baseline"),
                     default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
                          "(profiling workload); copy: K device copies of 256 MiB (PMC calibration); "
-                         "mixed / uneven / fwi / mg / mg-sources: one of the side legs alone (profiling workloads)")
+                         "mixed / uneven / fwi / mg / mg-sources: one of the side legs alone (profiling workloads); "
+                         "cpu-baseline: the CPU part of the baseline alone (a child process of the default run)")
     args = ap.parse_args()
-    _omp_binding()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU: the hot path is one solve per GPU and this build runs on a single "
@@ -533,6 +575,8 @@ def main():
               f"is not built.", file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "cpu-baseline":
+        return cpu_mode(args)  # no torch in this process (see _cpu_env)
     sys.path.insert(0, PKG)
     import numpy as np
     import torch
@@ -557,7 +601,7 @@ def main():
             else:
                 res = mg_leg(plain_rhs_per_s)
         else:
-            res = dict(zip(("grads_per_min", "fwi"), fwi_leg()))
+            res = dict(zip(("grads_per_min", "fwi"), fwi_leg(workers=args.fwi_workers)))
         print(json.dumps({"mode": args.mode, **res}))
         return
     cfg = config_c2()
@@ -691,7 +735,7 @@ def main():
         out["fwi_grads_per_min"], out["fwi"] = fwi_leg()
         print(f"[bench] fwi leg done: {out['fwi_grads_per_min']:.3f} grads/min", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, wn, iters, args.cpu_sample_iters)
+        out["cpu_baseline"] = cpu_baseline(iters, args.cpu_sample_iters)
         print(f"[bench] cpu baseline done: C1 {out['cpu_baseline']['c1']['cpu_seconds']:.1f} s on "
               f"{out['cpu_baseline']['cores']} threads", file=sys.stderr, flush=True)
     print(json.dumps(out))

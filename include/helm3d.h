@@ -136,21 +136,25 @@ int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *
                        const void *x_dev, void *y_dev);
 
 /* Stencil launch shape: z planes per block (0: automatic), column-tile rows
- * (4 or 8: 256 or 512 threads) and planes of loads in flight (1 to 4; the
- * batched kernels use at most 2).  The
- * result is identical for every shape; only speed changes.  Chosen per grid by
- * the autotuner of the Python package (helm3d.tune). */
-int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch);
-int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *blocks);
+ * (4 or 8), planes of loads in flight (1 to 4; the batched kernels use at
+ * most 2) and rows per thread (1, or 2 with 8 tile rows: 256 threads per
+ * 64 x 8 tile instead of 512, so two tiles share a compute unit).  The
+ * result is identical for every shape; only speed changes.  Chosen per grid
+ * by the autotuner (helm3d_autotune); the batched kernels run one row per
+ * thread. */
+int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch, int rows_per_thread);
+int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *rows_per_thread,
+                      int *blocks);
 /* Autotuner of the stencil launch shape for this context's grid: times every
- * candidate shape (64 x {4, 8} column tiles, prefetch {1, 2, 3, 4}, z chunks of the
- * whole column down to 8 planes) on the operator of `mode` (1: H, else H^H)
- * in `precision`, `reps` launches each with HIP events on the context's
- * stream, and keeps the fastest (results are identical for every shape).  No
- * counterpart in the reference; the batched solves use each item's tuned z
- * chunk and the first item's tile shape. */
+ * candidate shape (64 x 4 and 64 x 8 column tiles, the latter with one or two
+ * rows per thread, prefetch {1, 2, 3, 4}, z chunks of the whole column down
+ * to 8 planes) on the operator of `mode` (1: H, else H^H) in `precision`,
+ * `reps` launches each with HIP events on the context's stream, and keeps the
+ * fastest (results are identical for every shape).  No counterpart in the
+ * reference; the batched solves use each item's tuned z chunk and the first
+ * item's tile rows and prefetch. */
 int helm3d_autotune(helm3d_ctx *ctx, int mode, int precision, int reps, int *kchunk, int *tile_rows,
-                    int *prefetch, double *best_ms);
+                    int *prefetch, int *rows_per_thread, double *best_ms);
 
 /* Context options.  HELM3D_OPT_GRAPHS (default 1): replay fixed kernel
  * sequences from captured HIP graphs -- the fixed-length inner solve of

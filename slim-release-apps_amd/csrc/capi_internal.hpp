@@ -159,6 +159,8 @@ struct helm3d_ctx {
     size_t gs_cap = 0;
     double* nrm = nullptr;  // device: kMaxBasis + 1 values
     size_t nrm_cap = 0;
+    GmresDevState* gst = nullptr;  // device state of the result-free solves
+    size_t gst_cap = 0;
     double* ring_host = nullptr;  // pinned: one row per step of a cycle
     hipEvent_t ring_ev[kMaxBasis + 1] = {};
     // device addresses of the pinned mirrors: the reduction kernels write
@@ -196,7 +198,7 @@ struct helm3d_ctx {
     {
         (void)hipSetDevice(device);
         void* bufs[] = {tab64, tab32, wn64, wn32, part, small, sink, V64.p, Z64.p, T64.p, BC64.p,
-                        V32.p, Z32.p, T32a.p, T32b.p, BC32.p, bsc, gs, nrm};
+                        V32.p, Z32.p, T32a.p, T32b.p, BC32.p, bsc, gs, nrm, gst};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (small_host) (void)hipHostFree(small_host);
@@ -305,6 +307,9 @@ constexpr int kGsRow = 2 * kMaxBasis + 2;  // doubles per Arnoldi-step row
 // grids of at most this many points enqueue a whole Arnoldi cycle ahead of
 // the host (gmres_engine without a preconditioner)
 constexpr long kDeepLookahead = 2L << 20;
+// result-free calls without a preconditioner and with at most this many
+// restart cycles run all their decisions on the device (gmres_engine)
+constexpr long kMaxDeviceCycles = 16;
 
 // Context work buffers: reduction partials, reduced values (+ pinned host
 // mirror) and the stencil's write-only sink.
@@ -320,6 +325,7 @@ inline int ensure_work(helm3d_ctx* c)
     }
     CHK(ensure_t(&c->gs, &c->gs_cap, size_t(kMaxBasis + 1) * kGsRow));
     CHK(ensure_t(&c->nrm, &c->nrm_cap, size_t(kMaxBasis + 1)));
+    CHK(ensure_t(&c->gst, &c->gst_cap, 1));
     if (!c->ring_host) {
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->ring_host), (kMaxBasis + 1) * kGsRow * sizeof(double)));
         HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->ring_dev), c->ring_host, 0));

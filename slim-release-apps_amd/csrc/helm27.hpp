@@ -45,19 +45,22 @@ struct StencilLaunch {
     const cpx<R>* rh;
     double* sc;
     int rho_new, rho_old, first;
+    const int* skip;  // optional (device): the launch does nothing while *skip != 0
 };
 
 // Launch shape: 64 x ty column tiles, z chunks of kchunk planes, pf planes of
-// loads in flight.
+// loads in flight, ry rows per thread (2 only with ty = 8: 256 threads per
+// tile instead of 512).
 constexpr int kSinkSlots = 65536;
 
 struct StencilGrid {
-    int tiles_x, tiles_y, kchunk, nchunks, blocks, ty, pf;
+    int tiles_x, tiles_y, kchunk, nchunks, blocks, ty, pf, ry;
 };
 
-// kchunk <= 0 picks a chunk giving ~2048 blocks; ty in {4, 8}; pf in {1, 2}.
+// kchunk <= 0 picks a chunk giving ~480 blocks; ty in {4, 8}; pf in 1..4;
+// ry in {1, 2} (2 needs ty = 8; otherwise 1 is used).
 template <typename R>
-StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty = 4, int pf = 2);
+StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty = 4, int pf = 2, int ry = 1);
 
 template <typename R>
 hipError_t launch_stencil(int variant, StencilLaunch<R> L, const StencilGrid& g, hipStream_t s);

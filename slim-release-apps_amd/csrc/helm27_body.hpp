@@ -50,16 +50,18 @@ struct Cd {
     double re, im;
 };
 __device__ inline Cd cd_ld(const double* s, int off) { return Cd{s[off], s[off + 1]}; }
-__device__ inline Cd cd_mul(Cd a, Cd b) { return Cd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ inline Cd cd_mul(Cd a, Cd b) { return Cd{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)}; }
 __device__ inline Cd cd_div(Cd a, Cd b)
 {
-    const double d = b.re * b.re + b.im * b.im;
+    const double d = fma(b.re, b.re, b.im * b.im);
     if (d == 0.0) return Cd{0.0, 0.0};
-    return Cd{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+    return Cd{fma(a.re, b.re, a.im * b.im) / d, fma(a.im, b.re, -(a.re * b.im)) / d};
 }
 
-// TY: rows of the column tile (block = 64 x TY threads); PF: planes of global
-// loads kept in flight ahead of the plane being computed (1 or 2).
+// TY: rows of the column tile; RY: rows per thread (1 or 2: a block is
+// 64 x TY / RY threads, one wave per RY consecutive rows, and the 3 x (RY+2)
+// window of a plane is read once for the RY outputs); PF: planes of global
+// loads kept in flight ahead of the plane being computed (1 to 4).
 //
 // Memory pipeline: every global load of the loop goes into one of PF register
 // sets that are refilled right after being staged into LDS, so no register is
@@ -81,15 +83,21 @@ __device__ inline Cd cd_div(Cd a, Cd b)
 //      EPI 4 launch
 //   4: x = s = r - alpha v formed while staging, y = t = H s, partials
 //      (t, s), ||t||^2, (rhat, s) and (rhat, t)
+// (EPI 3 and 4 run one row per thread.)
 // The body runs one block `lin` of a launch of `G` blocks described by L.
-template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI>
+template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI, int RY = 1>
 __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int lin, const int G)
 {
-    constexpr int NT = TX * TY;
+    static_assert(RY == 1 || RY == 2, "one or two rows per thread");
+    static_assert(TY % RY == 0, "rows per thread divide the tile rows");
+    static_assert(RY == 1 || EPI <= 2, "the fused BiCGStab forms run one row per thread");
+    constexpr int NT = TX * TY / RY;     // threads
     constexpr int HY = TY + 2;
-    constexpr int NS = HX * HY;
-    static_assert(NS <= 2 * NT, "halo tile must fit two load slots per thread");
+    constexpr int NS = HX * HY;          // halo tile slots
+    constexpr int NSL = (NS + NT - 1) / NT;  // load slots per thread
+    static_assert(EPI < 3 || NSL == 2, "fused BiCGStab staging assumes two slots per thread");
     static_assert(PF >= 1 && PF <= 4, "prefetch depth 1 to 4");
+    if (L.skip && *L.skip) return;  // uniform over the launch
     const OpDesc<R>& op = L.op;
     __shared__ cpx<R> su[2][HY][HX];            // input vector plane (+halo)
     // forward mass operand: wn .* x (WNC, DERIV) or the real wn itself, the
@@ -120,15 +128,25 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     const int k1 = min(nz, k0 + L.kchunk);
 
     const int tid = threadIdx.x;
-    const int tx = tid & (TX - 1), ty = tid / TX;
-    const int i = i0 + tx, j = j0 + ty;
-    const bool active = (i < nx) && (j < ny);
-    const int ic = min(i, nx - 1), jc = min(j, ny - 1);  // clamped for table reads
-    // A wave is one row j of 64 columns: it takes the interior path when every
-    // column and the row have xi_lo = xi_hi = 1 at index-1, index, index+1.
-    const int jw = __builtin_amdgcn_readfirstlane(j);
-    const bool interior = i0 >= op.int_lo[0] && i0 + TX - 1 <= op.int_hi[0] && jw >= op.int_lo[1] &&
-                          jw <= op.int_hi[1];
+    const int tx = tid & (TX - 1), tw = tid / TX;  // lane, wave
+    const int tr0 = tw * RY;                       // first tile row of this thread
+    const int i = i0 + tx;
+    const int ic = min(i, nx - 1);  // clamped for table reads
+    // A wave is RY rows of 64 columns.  Per row it takes one of four paths
+    // (uniform over the wave, and the same for an output point whatever the
+    // launch shape): interior when every column and the row have
+    // xi_lo = xi_hi = 1 at index-1, index, index+1; x PML only or y PML only
+    // when one axis has; the general path otherwise.
+    const int jw = __builtin_amdgcn_readfirstlane(j0 + tr0);
+    const bool xint = i0 >= op.int_lo[0] && i0 + TX - 1 <= op.int_hi[0];
+    bool yint[RY], active[RY];
+    long offc[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        yint[r] = jw + r >= op.int_lo[1] && jw + r <= op.int_hi[1];
+        active[r] = (i < nx) && (jw + r < ny);
+        offc[r] = long(ic) + long(min(jw + r, ny - 1)) * nx;
+    }
 
     // ---- per-column coefficients (fixed for the whole march)
     // Every F and G coefficient is a real combination of the six x/y PML
@@ -137,18 +155,26 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // factored form: six complex multipliers times real-weighted sums of the
     // plane's 3x3 neighbourhood (24 registers instead of 18 complex
     // coefficients).  Adjoint: side values of the neighbours, conjugated.
-    // the y multipliers are uniform over the wave (one row j): read them at a
-    // wave-uniform index so they live in scalar registers
-    const int jcw = min(jw, ny - 1);
-    cpx<R> px0{}, px2{}, py0{}, py2{}, pxf{}, pyf{};
+    // The y multipliers are uniform over the wave (rows jw .. jw+RY-1): read
+    // at wave-uniform indices so they live in scalar registers.
+    cpx<R> px0{}, px2{}, pxf{};
+    cpx<R> py0[RY], py2[RY], pyf[RY];
     if constexpr (!DERIV) {
         if constexpr (!ADJ) {
             px0 = op.xlo[ic + 1]; px2 = op.xhi[ic + 1]; pxf = px0 + px2;
-            py0 = op.ylo[jcw + 1]; py2 = op.yhi[jcw + 1]; pyf = py0 + py2;
         } else {
             // neighbour i-1 couples through its hi value, i+1 through its lo value
             px0 = conjc(op.xhi[ic]); px2 = conjc(op.xlo[ic + 2]); pxf = conjc(op.xlo[ic + 1] + op.xhi[ic + 1]);
-            py0 = conjc(op.yhi[jcw]); py2 = conjc(op.ylo[jcw + 2]); pyf = conjc(op.ylo[jcw + 1] + op.yhi[jcw + 1]);
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const int jcw = min(jw + r, ny - 1);
+            if constexpr (!ADJ) {
+                py0[r] = op.ylo[jcw + 1]; py2[r] = op.yhi[jcw + 1]; pyf[r] = py0[r] + py2[r];
+            } else {
+                py0[r] = conjc(op.yhi[jcw]); py2[r] = conjc(op.ylo[jcw + 2]);
+                pyf[r] = conjc(op.ylo[jcw + 1] + op.yhi[jcw + 1]);
+            }
         }
         // z weights of this chunk: slots k0 .. k1+1 (planes k0-1 .. k1) plus
         // the padding entries read by the trailing pipeline steps
@@ -159,23 +185,31 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         }
     }
 
-    // ---- load slots: the (TX+2) x (TY+2) halo tile, two slots per thread
+    // ---- load slots: the (TX+2) x (TY+2) halo tile, NSL slots per thread
     const long plane = long(nx) * ny;
-    const int s0 = tid, s1 = tid + NT;
-    const int s0x = s0 % HX, s0y = s0 / HX;
-    const int s1x = s1 % HX, s1y = s1 / HX;
-    const bool has1 = s1 < NS;
-    const int g0i = i0 - 1 + s0x, g0j = j0 - 1 + s0y;
-    const int g1i = i0 - 1 + s1x, g1j = j0 - 1 + s1y;
-    const bool in0 = g0i >= 0 && g0i < nx && g0j >= 0 && g0j < ny;
-    const bool in1 = has1 && g1i >= 0 && g1i < nx && g1j >= 0 && g1j < ny;
-    const long off0 = in0 ? g0i + long(g0j) * nx : 0;
-    const long off1 = in1 ? g1i + long(g1j) * nx : 0;
-    const long offc = long(ic) + long(jc) * nx;
+    int sxs[NSL], sys[NSL];
+    bool sin_[NSL], shas[NSL];
+    long soff[NSL];
+#pragma unroll
+    for (int t = 0; t < NSL; ++t) {
+        const int sl = tid + t * NT;
+        sxs[t] = sl % HX;
+        sys[t] = sl / HX;
+        shas[t] = (t + 1) * NT <= NS || sl < NS;
+        const int gi = i0 - 1 + sxs[t], gj = j0 - 1 + sys[t];
+        sin_[t] = shas[t] && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+        soff[t] = sin_[t] ? gi + long(gj) * nx : 0;
+    }
     // gradient epilogue: position of the own column in the physical box
-    const int qi = i - L.p0[0], qj = j - L.p0[1];
-    const bool gcol = EPI == 2 && active && qi >= 0 && qi < L.pn[0] && qj >= 0 && qj < L.pn[1];
-    const long gcol_off = gcol ? qi + long(qj) * L.pn[0] : 0;
+    const int qi = i - L.p0[0];
+    bool gcol[RY];
+    long gcol_off[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int qj = jw + r - L.p0[1];
+        gcol[r] = EPI == 2 && active[r] && qi >= 0 && qi < L.pn[0] && qj >= 0 && qj < L.pn[1];
+        gcol_off[r] = gcol[r] ? qi + long(qj) * L.pn[0] : 0;
+    }
     const long gplane = long(L.pn[0]) * L.pn[1];
     // real-wn loads (forward); with a constant wn they read a valid dummy address
     const R* wnp = op.wn ? op.wn : reinterpret_cast<const R*>((EPI == 3 || EPI == 4) ? L.f0 : L.x);
@@ -242,276 +276,299 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // applied when the set is consumed, so the number of loads in flight is
     // fixed and the compiler waits only for the set it stages.
     struct Set {
-        cpx<R> x0, x1;  // slots (EPI 3/4: r)
-        cpx<R> e0, e1;  // EPI 3: p, EPI 4: v at the slots
-        cpx<R> f0, f1;  // EPI 3: v at the slots
-        cpx<R> g0, g1;  // EPI 3: t at the slots
-        cpx<R> zo;      // EPI 3: z at own point of plane q-1
-        R w0, w1;       // forward, real wn at the slots
-        cpx<R> c0, c1;  // forward, complex wn at the slots
-        cpx<R> wc;      // adjoint: wn at own point of plane q-1
-        cpx<R> bb;      // addend b at own point of plane q-1
-        cpx<R> gu;      // gradient: U at own point of plane q-1
-        double gv;      // gradient: g at own point of plane q-1
+        cpx<R> x[NSL];   // slots (EPI 3/4: r)
+        cpx<R> e[NSL];   // EPI 3: p, EPI 4: v at the slots
+        cpx<R> f[NSL];   // EPI 3: v at the slots
+        cpx<R> g[NSL];   // EPI 3: t at the slots
+        cpx<R> zo;       // EPI 3: z at own point of plane q-1
+        R w[NSL];        // forward, real wn at the slots
+        cpx<R> c[NSL];   // forward, complex wn at the slots
+        cpx<R> wc[RY];   // adjoint: wn at own points of plane q-1
+        cpx<R> bb[RY];   // addend b (EPI 1) / rhat (EPI 3/4) at own points of plane q-1
+        cpx<R> gu[RY];   // gradient: U at own points of plane q-1
+        double gv[RY];   // gradient: g at own points of plane q-1
     };
     auto fetch = [&](int q, Set& S) {
         const long base = long(min(max(q, 0), nz - 1)) * plane;
-        if constexpr (EPI == 3 || EPI == 4) {
-            S.x0 = L.f0[base + off0];
-            S.x1 = L.f0[base + off1];
-            S.e0 = L.f1[base + off0];
-            S.e1 = L.f1[base + off1];
-            if constexpr (EPI == 3) {
-                S.f0 = L.f2[base + off0];
-                S.f1 = L.f2[base + off1];
-                S.g0 = L.f3[base + off0];
-                S.g1 = L.f3[base + off1];
-            }
-        } else {
-            S.x0 = L.x[base + off0];
-            S.x1 = L.x[base + off1];
-        }
-        if constexpr (!ADJ) {
-            if constexpr (WNC) {
-                S.c0 = op.wnc[base + off0];
-                S.c1 = op.wnc[base + off1];
+#pragma unroll
+        for (int t = 0; t < NSL; ++t) {
+            if constexpr (EPI == 3 || EPI == 4) {
+                S.x[t] = L.f0[base + soff[t]];
+                S.e[t] = L.f1[base + soff[t]];
+                if constexpr (EPI == 3) {
+                    S.f[t] = L.f2[base + soff[t]];
+                    S.g[t] = L.f3[base + soff[t]];
+                }
             } else {
-                S.w0 = wnp[base + off0];
-                S.w1 = wnp[base + off1];
+                S.x[t] = L.x[base + soff[t]];
+            }
+            if constexpr (!ADJ) {
+                if constexpr (WNC) S.c[t] = op.wnc[base + soff[t]];
+                else S.w[t] = wnp[base + soff[t]];
             }
         }
-        const long own = long(min(max(q - 1, 0), nz - 1)) * plane + offc;
-        if constexpr (ADJ) {
-            if constexpr (WNC) S.wc = op.wnc[own];
-            else S.wc = cpx<R>{wnp[own], R(0)};
-        }
-        if constexpr (EPI == 1) S.bb = L.b[own];
-        if constexpr (EPI == 3 || EPI == 4) S.bb = L.rh[own];
-        if constexpr (EPI == 3) S.zo = L.zz[own];
-        if constexpr (EPI == 2) {
-            S.gu = L.gu[own];
-            const int qk = min(max(q - 1 - L.p0[2], 0), L.pn[2] - 1);
-            S.gv = L.gg[gcol_off + long(qk) * gplane];
+        const long ownk = long(min(max(q - 1, 0), nz - 1)) * plane;
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const long own = ownk + offc[r];
+            if constexpr (ADJ) {
+                if constexpr (WNC) S.wc[r] = op.wnc[own];
+                else S.wc[r] = cpx<R>{wnp[own], R(0)};
+            }
+            if constexpr (EPI == 1) S.bb[r] = L.b[own];
+            if constexpr (EPI == 3 || EPI == 4) S.bb[r] = L.rh[own];
+            if constexpr (EPI == 3) S.zo = L.zz[own];
+            if constexpr (EPI == 2) {
+                S.gu[r] = L.gu[own];
+                const int qk = min(max(q - 1 - L.p0[2], 0), L.pn[2] - 1);
+                S.gv[r] = L.gg[gcol_off[r] + long(qk) * gplane];
+            }
         }
     };
 
-    // pipeline state (see the header comment of this file)
-    cpx<R> Qprev = zero<R>(), Qcur = zero<R>(), Acc = zero<R>(), Pend = zero<R>();
-    cpx<R> MAcc = zero<R>(), MOcur = zero<R>();
+    // pipeline state per output row (see the header comment of this file)
+    cpx<R> Qprev[RY], Qcur[RY], Acc[RY], Pend[RY], MAcc[RY], MOcur[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        Qprev[r] = zero<R>();
+        Qcur[r] = zero<R>();
+        Acc[r] = zero<R>();
+        Pend[r] = zero<R>();
+        MAcc[r] = zero<R>();
+        MOcur[r] = zero<R>();
+    }
 
     // Stage set S (plane q) into LDS, refill S with plane q+PF, compute plane
     // q and emit plane q-1.
-    auto step = [&](int q, int t, Set& S) {
-        const int buf = t & 1;
+    auto step = [&](int q, int tstep, Set& S) {
+        const int buf = tstep & 1;
         const bool inz = q >= 0 && q < nz;
-        const bool v0 = inz && in0, v1 = inz && in1;
-        cpx<R> x0 = S.x0, x1 = S.x1;
-        if constexpr (EPI == 3) {
-            const cpx<R> s0 = cfma(S.x0, na, S.f0), s1 = cfma(S.x1, na, S.f1);
-            const cpx<R> r0 = cfma(s0, nw, S.g0), r1 = cfma(s1, nw, S.g1);
-            x0 = cfma(cfma(r0, bt, S.e0), nbw, S.f0);
-            x1 = cfma(cfma(r1, bt, S.e1), nbw, S.f1);
-            sr[buf][s0y][s0x] = r0;
-            sd[buf][s0y][s0x] = cfma(cmul(za, S.e0), zw, s0);
-            if (has1) {
-                sr[buf][s1y][s1x] = r1;
-                sd[buf][s1y][s1x] = cfma(cmul(za, S.e1), zw, s1);
+        cpx<R> xs[NSL];
+#pragma unroll
+        for (int t = 0; t < NSL; ++t) {
+            const bool v = inz && sin_[t];
+            cpx<R> xv = S.x[t];
+            if constexpr (EPI == 3) {
+                const cpx<R> sv = cfma(S.x[t], na, S.f[t]);
+                const cpx<R> rv = cfma(sv, nw, S.g[t]);
+                xv = cfma(cfma(rv, bt, S.e[t]), nbw, S.f[t]);
+                if (shas[t]) {
+                    sr[buf][sys[t]][sxs[t]] = rv;
+                    sd[buf][sys[t]][sxs[t]] = cfma(cmul(za, S.e[t]), zw, sv);
+                }
+            } else if constexpr (EPI == 4) {
+                xv = cfma(xv, ca, S.e[t]);
             }
-        } else if constexpr (EPI == 4) {
-            x0 = cfma(x0, ca, S.e0);
-            x1 = cfma(x1, ca, S.e1);
-        }
-        x0 = v0 ? x0 : zero<R>();
-        x1 = v1 ? x1 : zero<R>();
-        su[buf][s0y][s0x] = x0;
-        if (has1) su[buf][s1y][s1x] = x1;
-        if constexpr (!ADJ) {
-            if constexpr (WNC) {
-                sm[buf][s0y][s0x] = cmul(S.c0, x0);
-                if (has1) sm[buf][s1y][s1x] = cmul(S.c1, x1);
-            } else {
-                const R w0 = op.wn ? S.w0 : op.wn_const, w1 = op.wn ? S.w1 : op.wn_const;
-                if constexpr (kWnSep) {
-                    sw[buf][s0y][s0x] = w0;
-                    if (has1) sw[buf][s1y][s1x] = w1;
+            xs[t] = v ? xv : zero<R>();
+            if (shas[t]) su[buf][sys[t]][sxs[t]] = xs[t];
+            if constexpr (!ADJ) {
+                if constexpr (WNC) {
+                    if (shas[t]) sm[buf][sys[t]][sxs[t]] = cmul(S.c[t], xs[t]);
                 } else {
-                    sm[buf][s0y][s0x] = w0 * x0;
-                    if (has1) sm[buf][s1y][s1x] = w1 * x1;
+                    const R wv = op.wn ? S.w[t] : op.wn_const;
+                    if constexpr (kWnSep) {
+                        if (shas[t]) sw[buf][sys[t]][sxs[t]] = wv;
+                    } else {
+                        if (shas[t]) sm[buf][sys[t]][sxs[t]] = wv * xs[t];
+                    }
                 }
             }
         }
         // own-point operands of plane q-1 leave the set before the refill
-        cpx<R> wc = S.wc;
-        if constexpr (ADJ && !WNC) wc.re = op.wn ? wc.re : op.wn_const;
-        const cpx<R> bb = S.bb, gu = S.gu, zo = S.zo;
-        const double gv = S.gv;
+        cpx<R> wc[RY], bb[RY], gu[RY];
+        double gv[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            wc[r] = S.wc[r];
+            if constexpr (ADJ && !WNC) wc[r].re = op.wn ? wc[r].re : op.wn_const;
+            bb[r] = S.bb[r];
+            gu[r] = S.gu[r];
+            gv[r] = S.gv[r];
+        }
+        const cpx<R> zo = S.zo;
         fetch(q + PF, S);  // unconditional: a fixed number of loads per step
         __syncthreads();
 
-        // ---- the plane's 3x3 neighbourhood (stiffness operand)
-        cpx<R> u[3][3];
-        if constexpr (!DERIV) {
+        // ---- the plane's 3 x (RY+2) window (stiffness operand) and its
+        // mass operand; b indexes the rows tr0-1 .. tr0+RY of the tile
+        constexpr int WB = RY + 2;
+        // u: the window; m1: the mass operand of the middle column; xsu, xsm:
+        // row sums along x of u and of the mass operand, shared by the RY
+        // outputs (the side columns of the mass operand are needed only
+        // through these sums)
+        cpx<R> u[3][WB], m1[WB], xsu[WB], xsm[WB];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < WB; ++b) {
+            cpx<R> uv[3];
 #pragma unroll
-                for (int b = 0; b < 3; ++b) u[a][b] = su[buf][ty + b][tx + a];
-        }
-
-        // ---- mass-lumping sums (forward: on wn .* x, adjoint: on x)
-        cpx<R> Mc, Mo;
-        {
-            cpx<R> m[3][3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    if constexpr (kWnSep) m[a][b] = sw[buf][ty + b][tx + a] * u[a][b];
-                    else if constexpr (ADJ) m[a][b] = DERIV ? su[buf][ty + b][tx + a] : u[a][b];
-                    else m[a][b] = sm[buf][ty + b][tx + a];
-                }
-            const cpx<R> mf = (m[0][1] + m[2][1]) + (m[1][0] + m[1][2]);
-            const cpx<R> md = (m[0][0] + m[2][0]) + (m[0][2] + m[2][2]);
-            Mc = op.wn_coef * m[1][1];
-            Mc = rfma(Mc, -op.wm2, mf);
-            Mc = rfma(Mc, -op.wm3, md);
-            Mo = (-op.wm2) * m[1][1];
-            Mo = rfma(Mo, -op.wm3, mf);
-            Mo = rfma(Mo, -op.wm4, md);
-        }
-
-        // ---- stiffness: Qz (z PML, applied per plane) and F, G (x/y PML)
-        cpx<R> Qn = zero<R>(), Fn = zero<R>(), Gn = zero<R>();
-        if constexpr (!DERIV) {
-            // pair sums along y (per column a) and along x (per row b)
-            const cpx<R> sy0 = u[0][0] + u[0][2], sy1 = u[1][0] + u[1][2], sy2 = u[2][0] + u[2][2];
-            const cpx<R> sx1 = u[0][1] + u[2][1];
-            const cpx<R> dg = sy0 + sy2;
-            Qn = op.c[2] * u[1][1];
-            Qn = rfma(Qn, -op.xz, sx1);
-            Qn = rfma(Qn, -op.yz, sy1);
-            Qn = rfma(Qn, -op.w3a2, dg);
-            if (interior) {
-                // x/y PML absent around this wave: F, G are real constant
-                // stencils on the sums already formed for Qz
-                Fn = op.fi[0] * u[1][1];
-                Fn = rfma(Fn, op.fi[1], sx1);
-                Fn = rfma(Fn, op.fi[2], sy1);
-                Fn = rfma(Fn, op.fi[3], dg);
-                Gn = op.gi[0] * u[1][1];
-                Gn = rfma(Gn, op.gi[1], sx1);
-                Gn = rfma(Gn, op.gi[2], sy1);
-                Gn = rfma(Gn, op.gi[3], dg);
+            for (int a = 0; a < 3; ++a) {
+                uv[a] = su[buf][tr0 + b][tx + a];
+                if constexpr (!DERIV) u[a][b] = uv[a];
+            }
+            if constexpr (!DERIV) xsu[b] = uv[0] + uv[2];
+            if constexpr (kWnSep) {
+                // wn .* x formed here: w0 x0 + w2 x2 as one product and one fma
+                const R w0 = sw[buf][tr0 + b][tx], w1 = sw[buf][tr0 + b][tx + 1], w2 = sw[buf][tr0 + b][tx + 2];
+                xsm[b] = rfma(w2 * uv[2], w0, uv[0]);
+                m1[b] = w1 * uv[1];
+            } else if constexpr (ADJ) {
+                xsm[b] = uv[0] + uv[2];
+                m1[b] = uv[1];
             } else {
-                // F: centre -(cx xf + cy yf); x faces cx xs + w_xy yf; y faces
-                //    cy ys + w_xy xf; diagonals -w_xy (xs + ys)
-                // G: centre w_xz xf + w_yz yf; x faces -w_xz xs + w3a2 yf;
-                //    y faces -w_yz ys + w3a2 xf; diagonals -w3a2 (xs + ys)
-                // (the non-xi_z part of Helm3d_27pt.h:680-714), grouped by
-                // multiplier; sy0/sy2: column sums, sx0/sx2: row sums of the
-                // diagonal pairs.
-                const cpx<R> sx0 = u[0][0] + u[2][0], sx2 = u[0][2] + u[2][2];
-                const R cx = op.c[0], cy = op.c[1], wxy = op.xy, wxz = op.xz, wyz = op.yz, w3 = op.w3a2;
-                // accumulated multiplier by multiplier (few live temporaries)
-                Fn = cmul(px0, rfma(cx * u[0][1], -wxy, sy0));
-                Gn = cmul(px0, rfma((-wxz) * u[0][1], -w3, sy0));
-                Fn = cfma(Fn, px2, rfma(cx * u[2][1], -wxy, sy2));
-                Gn = cfma(Gn, px2, rfma((-wxz) * u[2][1], -w3, sy2));
-                Fn = cfma(Fn, py0, rfma(cy * u[1][0], -wxy, sx0));
-                Gn = cfma(Gn, py0, rfma((-wyz) * u[1][0], -w3, sx0));
-                Fn = cfma(Fn, py2, rfma(cy * u[1][2], -wxy, sx2));
-                Gn = cfma(Gn, py2, rfma((-wyz) * u[1][2], -w3, sx2));
-                Fn = cfma(Fn, pyf, rfma(wxy * sx1, -cy, u[1][1]));
-                Gn = cfma(Gn, pyf, rfma(w3 * sx1, wyz, u[1][1]));
-                Fn = cfma(Fn, pxf, rfma(wxy * sy1, -cx, u[1][1]));
-                Gn = cfma(Gn, pxf, rfma(w3 * sy1, wxz, u[1][1]));
+                xsm[b] = sm[buf][tr0 + b][tx] + sm[buf][tr0 + b][tx + 2];
+                m1[b] = sm[buf][tr0 + b][tx + 1];
             }
         }
 
-        cpx<R> out = zero<R>();
         const bool emit = q >= k0 + 1 && q <= k1;
         const int kc = q - 1;
-        if constexpr (!ADJ) {
-            // y(kc) = wlo Q(kc-1) + whi Q(kc+1) - (wlo+whi) Q(kc) + GM(kc-1) + GM(kc+1) + FM(kc)
-            const cpx<R> GMn = Gn + Mo;
-            if (emit) {
-                out = Acc + GMn;
-                if constexpr (!DERIV) {
-                    const cpx<R> wl = szl[kc + 1 - k0], wh = szh[kc + 1 - k0];
-                    out = cfma(out, wl, Qprev - Qcur);
-                    out = cfma(out, wh, Qn - Qcur);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            // ---- mass-lumping sums (forward: on wn .* x, adjoint: on x)
+            const cpx<R> mf = xsm[r + 1] + (m1[r] + m1[r + 2]);
+            const cpx<R> md = xsm[r] + xsm[r + 2];
+            cpx<R> Mc = op.wn_coef * m1[r + 1];
+            Mc = rfma(Mc, -op.wm2, mf);
+            Mc = rfma(Mc, -op.wm3, md);
+            cpx<R> Mo = (-op.wm2) * m1[r + 1];
+            Mo = rfma(Mo, -op.wm3, mf);
+            Mo = rfma(Mo, -op.wm4, md);
+
+            // ---- stiffness: Qz (z PML, applied per plane) and F, G (x/y PML)
+            cpx<R> Qn = zero<R>(), Fn = zero<R>(), Gn = zero<R>();
+            if constexpr (!DERIV) {
+                const cpx<R> u11 = u[1][r + 1];
+                // pair sums along y (column a) and along x (row b)
+                const cpx<R> sy1 = u[1][r] + u[1][r + 2];
+                const cpx<R> sx0 = xsu[r], sx1 = xsu[r + 1], sx2 = xsu[r + 2];
+                const cpx<R> dg = sx0 + sx2;
+                Qn = op.c[2] * u11;
+                Qn = rfma(Qn, -op.xz, sx1);
+                Qn = rfma(Qn, -op.yz, sy1);
+                Qn = rfma(Qn, -op.w3a2, dg);
+                const R cx = op.c[0], cy = op.c[1], wxy = op.xy, wxz = op.xz, wyz = op.yz, w3 = op.w3a2;
+                if (xint && yint[r]) {
+                    // x/y PML absent around this wave: F, G are real constant
+                    // stencils on the sums already formed for Qz
+                    Fn = op.fi[0] * u11;
+                    Fn = rfma(Fn, op.fi[1], sx1);
+                    Fn = rfma(Fn, op.fi[2], sy1);
+                    Fn = rfma(Fn, op.fi[3], dg);
+                    Gn = op.gi[0] * u11;
+                    Gn = rfma(Gn, op.gi[1], sx1);
+                    Gn = rfma(Gn, op.gi[2], sy1);
+                    Gn = rfma(Gn, op.gi[3], dg);
+                } else {
+                    // F: centre -(cx xf + cy yf); x faces cx xs + w_xy yf; y faces
+                    //    cy ys + w_xy xf; diagonals -w_xy (xs + ys)
+                    // G: centre w_xz xf + w_yz yf; x faces -w_xz xs + w3a2 yf;
+                    //    y faces -w_yz ys + w3a2 xf; diagonals -w3a2 (xs + ys)
+                    // (the non-xi_z part of Helm3d_27pt.h:680-714), grouped by
+                    // multiplier; sy0/sy2: column sums, sx0/sx2: row sums of the
+                    // diagonal pairs.
+                    const cpx<R> sy0 = u[0][r] + u[0][r + 2], sy2 = u[2][r] + u[2][r + 2];
+                    // accumulated multiplier by multiplier (few live temporaries)
+                    Fn = cmul(px0, rfma(cx * u[0][r + 1], -wxy, sy0));
+                    Gn = cmul(px0, rfma((-wxz) * u[0][r + 1], -w3, sy0));
+                    Fn = cfma(Fn, px2, rfma(cx * u[2][r + 1], -wxy, sy2));
+                    Gn = cfma(Gn, px2, rfma((-wxz) * u[2][r + 1], -w3, sy2));
+                    Fn = cfma(Fn, py0[r], rfma(cy * u[1][r], -wxy, sx0));
+                    Gn = cfma(Gn, py0[r], rfma((-wyz) * u[1][r], -w3, sx0));
+                    Fn = cfma(Fn, py2[r], rfma(cy * u[1][r + 2], -wxy, sx2));
+                    Gn = cfma(Gn, py2[r], rfma((-wyz) * u[1][r + 2], -w3, sx2));
+                    Fn = cfma(Fn, pyf[r], rfma(wxy * sx1, -cy, u11));
+                    Gn = cfma(Gn, pyf[r], rfma(w3 * sx1, wyz, u11));
+                    Fn = cfma(Fn, pxf, rfma(wxy * sy1, -cx, u11));
+                    Gn = cfma(Gn, pxf, rfma(w3 * sy1, wxz, u11));
                 }
             }
-            Qprev = Qcur;
-            Qcur = Qn;
-            Acc = Pend + (Fn + Mc);
-            Pend = GMn;
-        } else {
-            // contributions of plane q to outputs q+1 (Sdn), q-1 (Sup), q (S0)
-            cpx<R> Sdn = Gn, Sup = Gn, S0 = Fn;
-            if constexpr (!DERIV) {
-                const cpx<R> zl = szl[q + 1 - k0], zh = szh[q + 1 - k0];
-                Sdn = cfma(Sdn, conjc(zh), Qn);
-                Sup = cfma(Sup, conjc(zl), Qn);
-                S0 = cfma(S0, -conjc(zl + zh), Qn);
+
+            cpx<R> out = zero<R>();
+            if constexpr (!ADJ) {
+                // y(kc) = wlo Q(kc-1) + whi Q(kc+1) - (wlo+whi) Q(kc) + GM(kc-1) + GM(kc+1) + FM(kc)
+                const cpx<R> GMn = Gn + Mo;
+                if (emit) {
+                    out = Acc[r] + GMn;
+                    if constexpr (!DERIV) {
+                        const cpx<R> wl = szl[kc + 1 - k0], wh = szh[kc + 1 - k0];
+                        out = cfma(out, wl, Qprev[r] - Qcur[r]);
+                        out = cfma(out, wh, Qn - Qcur[r]);
+                    }
+                }
+                Qprev[r] = Qcur[r];
+                Qcur[r] = Qn;
+                Acc[r] = Pend[r] + (Fn + Mc);
+                Pend[r] = GMn;
+            } else {
+                // contributions of plane q to outputs q+1 (Sdn), q-1 (Sup), q (S0)
+                cpx<R> Sdn = Gn, Sup = Gn, S0 = Fn;
+                if constexpr (!DERIV) {
+                    const cpx<R> zl = szl[q + 1 - k0], zh = szh[q + 1 - k0];
+                    Sdn = cfma(Sdn, conjc(zh), Qn);
+                    Sup = cfma(Sup, conjc(zl), Qn);
+                    S0 = cfma(S0, -conjc(zl + zh), Qn);
+                }
+                if (emit) {
+                    out = Acc[r] + Sup;
+                    out = cfma(out, conjc(wc[r]), MAcc[r] + Mo);
+                }
+                Acc[r] = Pend[r] + S0;
+                MAcc[r] = MOcur[r] + Mc;
+                Pend[r] = Sdn;
+                MOcur[r] = Mo;
             }
-            if (emit) {
-                out = Acc + Sup;
-                out = cfma(out, conjc(wc), MAcc + Mo);
+            // Exactly one store per step, lane and row (four for EPI 3, with
+            // one row): steps or lanes without an output write to the sink
+            // instead (keeps the vmcnt bookkeeping exact).
+            const bool live = emit && active[r];
+            const int sk = r == 0 ? sink_slot : sink_slot2;
+            if constexpr (EPI == 3) {
+                const long od = long(kc) * plane + offc[0];
+                cpx<R>* dy = live ? &L.y[od] : &L.sink[sink_slot];
+                cpx<R>* d2 = live ? &L.o2[od] : &L.sink[sink_slot2];
+                cpx<R>* d3 = live ? &L.o3[od] : &L.sink[sink_slot3];
+                cpx<R>* d4 = live ? &L.zz[od] : &L.sink[sink_slot4];
+                *dy = out;
+                *d2 = Xlast;
+                *d3 = Rlast;
+                *d4 = zo + Dlast;
+                if (live) {
+                    // (rhat, v) += conj(rhat) v
+                    acc[0] += double(bb[0].re) * double(out.re) + double(bb[0].im) * double(out.im);
+                    acc[1] += double(bb[0].re) * double(out.im) - double(bb[0].im) * double(out.re);
+                }
+                Xlast = su[buf][tw + 1][tx + 1];  // own point of plane q
+                Rlast = sr[buf][tw + 1][tx + 1];
+                Dlast = sd[buf][tw + 1][tx + 1];
+            } else if constexpr (EPI == 4) {
+                cpx<R>* dy = live ? &L.y[long(kc) * plane + offc[0]] : &L.sink[sink_slot];
+                *dy = out;
+                if (live) {
+                    // (t, s) += conj(t) s ; ||t||^2 ; (rhat, s) ; (rhat, t)
+                    const double tr_ = out.re, ti = out.im, sr_ = Xlast.re, si = Xlast.im, hr = bb[0].re,
+                                 hi = bb[0].im;
+                    acc[0] += tr_ * sr_ + ti * si;
+                    acc[1] += tr_ * si - ti * sr_;
+                    acc[2] += tr_ * tr_ + ti * ti;
+                    acc[3] += hr * sr_ + hi * si;
+                    acc[4] += hr * si - hi * sr_;
+                    acc[5] += hr * tr_ + hi * ti;
+                    acc[6] += hr * ti - hi * tr_;
+                }
+                Xlast = su[buf][tw + 1][tx + 1];  // own point of plane q
+            } else if constexpr (EPI == 2) {
+                const int qk = kc - L.p0[2];
+                const bool g = live && gcol[r] && qk >= 0 && qk < L.pn[2];
+                const double v = fma(double(gu[r].re), double(out.re), double(gu[r].im) * double(out.im));
+                double* dst = g ? &L.gg[gcol_off[r] + long(qk) * gplane] : reinterpret_cast<double*>(&L.sink[sk]);
+                *dst = fma(L.gscale, v, gv[r]);
+            } else {
+                cpx<R> res = alpha * out;
+                if constexpr (EPI == 1) res = rfma(res, L.beta, bb[r]);
+                cpx<R>* dst = live ? &L.y[long(kc) * plane + offc[r]] : &L.sink[sk];
+                *dst = res;
+                if (live) acc[0] += double(res.re) * double(res.re) + double(res.im) * double(res.im);
             }
-            Acc = Pend + S0;
-            MAcc = MOcur + Mc;
-            Pend = Sdn;
-            MOcur = Mo;
-        }
-        // Exactly one store per step and lane (two for EPI 3/4): steps or
-        // lanes without an output write to the sink instead (keeps the vmcnt
-        // bookkeeping exact).
-        const bool live = emit && active;
-        if constexpr (EPI == 3) {
-            const long od = long(kc) * plane + offc;
-            cpx<R>* dy = live ? &L.y[od] : &L.sink[sink_slot];
-            cpx<R>* d2 = live ? &L.o2[od] : &L.sink[sink_slot2];
-            cpx<R>* d3 = live ? &L.o3[od] : &L.sink[sink_slot3];
-            cpx<R>* d4 = live ? &L.zz[od] : &L.sink[sink_slot4];
-            *dy = out;
-            *d2 = Xlast;
-            *d3 = Rlast;
-            *d4 = zo + Dlast;
-            if (live) {
-                // (rhat, v) += conj(rhat) v
-                acc[0] += double(bb.re) * double(out.re) + double(bb.im) * double(out.im);
-                acc[1] += double(bb.re) * double(out.im) - double(bb.im) * double(out.re);
-            }
-            Xlast = su[buf][ty + 1][tx + 1];  // own point of plane q
-            Rlast = sr[buf][ty + 1][tx + 1];
-            Dlast = sd[buf][ty + 1][tx + 1];
-        } else if constexpr (EPI == 4) {
-            cpx<R>* dy = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
-            *dy = out;
-            if (live) {
-                // (t, s) += conj(t) s ; ||t||^2 ; (rhat, s) ; (rhat, t)
-                const double tr = out.re, ti = out.im, sr_ = Xlast.re, si = Xlast.im, hr = bb.re, hi = bb.im;
-                acc[0] += tr * sr_ + ti * si;
-                acc[1] += tr * si - ti * sr_;
-                acc[2] += tr * tr + ti * ti;
-                acc[3] += hr * sr_ + hi * si;
-                acc[4] += hr * si - hi * sr_;
-                acc[5] += hr * tr + hi * ti;
-                acc[6] += hr * ti - hi * tr;
-            }
-            Xlast = su[buf][ty + 1][tx + 1];  // own point of plane q
-        } else if constexpr (EPI == 2) {
-            const int qk = kc - L.p0[2];
-            const bool g = live && gcol && qk >= 0 && qk < L.pn[2];
-            const double v = double(gu.re) * double(out.re) + double(gu.im) * double(out.im);
-            double* dst = g ? &L.gg[gcol_off + long(qk) * gplane] : reinterpret_cast<double*>(&L.sink[sink_slot]);
-            *dst = gv + L.gscale * v;
-        } else {
-            cpx<R> r = alpha * out;
-            if constexpr (EPI == 1) r = rfma(r, L.beta, bb);
-            cpx<R>* dst = live ? &L.y[long(kc) * plane + offc] : &L.sink[sink_slot];
-            *dst = r;
-            if (live) acc[0] += double(r.re) * double(r.re) + double(r.im) * double(r.im);
         }
     };
 
@@ -529,10 +586,10 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         fetch(k0 - 1 + p, S[p]);
     }
     const int ntrips = (k1 - k0 + 2 + PF - 1) / PF;
-    for (int tr = 0; tr < ntrips; ++tr) {
-        const int q = k0 - 1 + tr * PF;
+    for (int trp = 0; trp < ntrips; ++trp) {
+        const int q = k0 - 1 + trp * PF;
 #pragma unroll
-        for (int p = 0; p < PF; ++p) step(q + p, tr * PF + p, S[p]);
+        for (int p = 0; p < PF; ++p) step(q + p, trp * PF + p, S[p]);
     }
 
     // optional per-block partials (deterministic two-stage reduction):

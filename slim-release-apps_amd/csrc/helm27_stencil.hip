@@ -18,8 +18,8 @@
 // The adjoint is the exact conjugate transpose: F/G take the neighbour's x/y
 // PML values and the z weights come from the neighbouring planes.
 //
-// Tiling.  A block of 64 x TY threads (TY = 4 or 8) owns a 64 x TY column tile
-// and marches over a z chunk.  Each plane (with a one-point x/y halo) is
+// Tiling.  A block of 64 x TY / RY threads (TY = 4 or 8 rows, RY = 1 or 2 rows
+// per thread) owns a 64 x TY column tile and marches over a z chunk.  Each plane (with a one-point x/y halo) is
 // staged once in LDS (double-buffered, one barrier per plane); the global
 // loads of the next PF planes (PF = 1..4) are in flight while a plane is
 // computed, so HBM latency overlaps the march.  Block
@@ -31,20 +31,21 @@ namespace helm3d {
 
 namespace {
 
-template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI>
-__global__ __launch_bounds__(TX* TY) void helm27_kernel(StencilLaunch<R> L)
+template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI, int RY>
+__global__ __launch_bounds__(TX* TY / RY) void helm27_kernel(StencilLaunch<R> L)
 {
-    helm27_body<R, ADJ, DERIV, WNC, TY, PF, EPI>(L, blockIdx.x, gridDim.x);
+    helm27_body<R, ADJ, DERIV, WNC, TY, PF, EPI, RY>(L, blockIdx.x, gridDim.x);
 }
 
 }  // namespace
 
 template <typename R>
-StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty, int pf)
+StencilGrid stencil_grid(int nx, int ny, int nz, int kchunk, int ty, int pf, int ry)
 {
     StencilGrid g;
     g.ty = (ty == 8) ? 8 : 4;
     g.pf = std::max(1, std::min(pf, 4));
+    g.ry = (ry == 2 && g.ty == 8) ? 2 : 1;
     g.tiles_x = (nx + TX - 1) / TX;
     g.tiles_y = (ny + g.ty - 1) / g.ty;
     if (kchunk <= 0) {
@@ -67,19 +68,26 @@ template <typename R, bool A, bool D, bool W, int EPI>
 hipError_t launch_shape(const StencilLaunch<R>& L, const StencilGrid& g, hipStream_t s)
 {
     const dim3 grid(g.blocks);
-    if (g.ty == 8) {
+    if (g.ty == 8 && g.ry == 2) {
         switch (g.pf) {
-        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 1, EPI>), grid, dim3(TX * 8), 0, s, L); break;
-        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 2, EPI>), grid, dim3(TX * 8), 0, s, L); break;
-        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 3, EPI>), grid, dim3(TX * 8), 0, s, L); break;
-        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 4, EPI>), grid, dim3(TX * 8), 0, s, L); break;
+        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 1, EPI, 2>), grid, dim3(TX * 4), 0, s, L); break;
+        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 2, EPI, 2>), grid, dim3(TX * 4), 0, s, L); break;
+        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 3, EPI, 2>), grid, dim3(TX * 4), 0, s, L); break;
+        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 4, EPI, 2>), grid, dim3(TX * 4), 0, s, L); break;
+        }
+    } else if (g.ty == 8) {
+        switch (g.pf) {
+        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 1, EPI, 1>), grid, dim3(TX * 8), 0, s, L); break;
+        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 2, EPI, 1>), grid, dim3(TX * 8), 0, s, L); break;
+        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 3, EPI, 1>), grid, dim3(TX * 8), 0, s, L); break;
+        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 8, 4, EPI, 1>), grid, dim3(TX * 8), 0, s, L); break;
         }
     } else {
         switch (g.pf) {
-        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 1, EPI>), grid, dim3(TX * 4), 0, s, L); break;
-        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 2, EPI>), grid, dim3(TX * 4), 0, s, L); break;
-        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 3, EPI>), grid, dim3(TX * 4), 0, s, L); break;
-        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 4, EPI>), grid, dim3(TX * 4), 0, s, L); break;
+        case 1: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 1, EPI, 1>), grid, dim3(TX * 4), 0, s, L); break;
+        case 2: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 2, EPI, 1>), grid, dim3(TX * 4), 0, s, L); break;
+        case 3: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 3, EPI, 1>), grid, dim3(TX * 4), 0, s, L); break;
+        default: hipLaunchKernelGGL((helm27_kernel<R, A, D, W, 4, 4, EPI, 1>), grid, dim3(TX * 4), 0, s, L); break;
         }
     }
     return hipGetLastError();
@@ -118,8 +126,8 @@ hipError_t launch_stencil(int variant, StencilLaunch<R> L, const StencilGrid& g,
     return launch_epi<R, true, true, true>(L, g, s);
 }
 
-template StencilGrid stencil_grid<double>(int, int, int, int, int, int);
-template StencilGrid stencil_grid<float>(int, int, int, int, int, int);
+template StencilGrid stencil_grid<double>(int, int, int, int, int, int, int);
+template StencilGrid stencil_grid<float>(int, int, int, int, int, int, int);
 template hipError_t launch_stencil<double>(int, StencilLaunch<double>, const StencilGrid&, hipStream_t);
 template hipError_t launch_stencil<float>(int, StencilLaunch<float>, const StencilGrid&, hipStream_t);
 

@@ -92,9 +92,10 @@ Vecs<float>& zbasis_of<float>(helm3d_ctx* c) { return c->Z32; }
 // Stencil launch on the context's operator; y = alpha*Hx + beta*b.
 template <typename R>
 int stencil(helm3d_ctx* c, int variant, const OpDesc<R>& op, const cpx<R>* x, cpx<R>* y, R alpha,
-            const cpx<R>* b, R beta, double* norm_part)
+            const cpx<R>* b, R beta, double* norm_part, const int* skip = nullptr)
 {
     StencilLaunch<R> L{};
+    L.skip = skip;
     L.op = op;
     L.x = x;
     L.y = y;
@@ -131,7 +132,8 @@ namespace helm3d {
 // exists for the current launch arguments: the operator descriptor, launch
 // shape, cycle length and every buffer address the steps use.
 template <typename R, typename Rec>
-int cycle_graph(helm3d_ctx* c, int variant, int m, const cpx<R>* V, long ld, int nblk, Rec&& record)
+int cycle_graph(helm3d_ctx* c, int variant, int m, const cpx<R>* V, long ld, int nblk, const int* skip,
+                Rec&& record)
 {
     const int p = std::is_same<R, double>::value ? 0 : 1;
     std::vector<unsigned char> key;
@@ -140,9 +142,9 @@ int cycle_graph(helm3d_ctx* c, int variant, int m, const cpx<R>* V, long ld, int
         key.insert(key.end(), b, b + n);
     };
     const OpDesc<R>& op = op_of<R>(c);
-    const void* ptrs[] = {V, c->part, c->nrm, c->gs, c->ring_dev, c->sink};
+    const void* ptrs[] = {V, c->part, c->nrm, c->gs, c->ring_dev, c->sink, skip};
     const long ints[] = {variant, m, ld, nblk, c->N, c->grid.tiles_x, c->grid.tiles_y, c->grid.kchunk,
-                         c->grid.blocks, c->grid.ty, c->grid.pf};
+                         c->grid.blocks, c->grid.ty, c->grid.pf, c->grid.ry};
     put(&op, sizeof(op));
     put(ptrs, sizeof(ptrs));
     put(ints, sizeof(ints));
@@ -192,6 +194,10 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
 {
     using C = std::complex<double>;
     if (m < 1 || m > kMaxBasis - 1) return fail(HELM3D_ERR_ARG, "maxinnerit must be in [1, 31]");
+    // Result-free calls without a preconditioner (multigrid smoothers and
+    // coarsest solves) run every decision on the device: no host round trip.
+    const bool device_driven =
+        !P && !need_result && !hst && iter_cap <= 0 && tol < 1.0 && maxit >= 1 && maxit <= kMaxDeviceCycles;
     const long N = c->N;
     const long ld = slot_ld(N);
     OpDesc<R>& op = op_of<R>(c);
@@ -207,6 +213,8 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     }
     const int nblk = krylov_blocks(N);
     auto slot = [&](int s) { return V + size_t(s) * ld; };
+    GmresDevState* st = c->gst;
+    const int* skip = device_driven ? &st->skip : nullptr;
 
     // r0 = b - A x0, or b with x = 0 (FGMRES.m:57-62); ||r0||^2 into nrm[0]
     // and the pinned mirror
@@ -219,6 +227,81 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
         TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm, c->stream, c->small_host_dev));
     }
+    // enqueue Arnoldi step j: slot(j) = A slot(j-1)/||slot(j-1)||, dots into
+    // row j, update with device scales, ||slot(j)||^2 into row j and nrm[j]
+    auto enqueue_on = [&](int j, hipStream_t s, bool record) -> int {
+        double* row = c->gs + size_t(j) * kGsRow;
+        StencilLaunch<R> L{};
+        L.op = op;
+        L.x = slot(j - 1);
+        L.y = slot(j);
+        L.alpha = R(1);
+        L.beta = R(0);
+        L.anrm = c->nrm + (j - 1);
+        L.sink = static_cast<cpx<R>*>(c->sink);
+        L.skip = skip;
+        TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, s));
+        // reduced dots into row (the update's coefficients) and the
+        // pinned ring row, ||slot(j)||^2 into nrm[j] and the ring row
+        double* hrow = device_driven ? row : c->ring_dev + size_t(j) * kGsRow;
+        if (nblk <= kInBlockDots) {
+            // short dots columns: summed by every block of the update
+            // pass (partials one column up, clear of the norm partials)
+            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part + nblk, nblk, s, skip));
+            TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, nullptr, CoefPack{}, slot(j), N, c->part,
+                                                       nblk, s, c->nrm, c->part + nblk, hrow, skip));
+        } else {
+            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, s, skip));
+            TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, s, hrow, skip));
+            TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk, s,
+                                                       c->nrm, nullptr, nullptr, skip));
+        }
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, s, hrow + 2 * j, skip));
+        if (record) HIPCHK(hipEventRecord(c->ring_ev[j], s));
+        return HELM3D_OK;
+    };
+
+    // a whole cycle of steps ahead of the host: replayed from a captured
+    // graph on small grids
+    auto enqueue_cycle = [&](bool use_graph) -> int {
+        if (use_graph)
+            return cycle_graph<R>(c, variant, m, V, ld, nblk, skip, [&](hipStream_t s) -> int {
+                for (int j = 1; j <= m; ++j) CHK(enqueue_on(j, s, false));
+                return HELM3D_OK;
+            });
+        for (int j = 1; j <= m; ++j) CHK(enqueue_on(j, c->stream, false));
+        return HELM3D_OK;
+    };
+
+    if (device_driven) {
+        // Every cycle the host loop below could run (floor(it/m) < maxit with
+        // m steps per cycle) is enqueued; the device takes its decisions: the
+        // least squares and the quasi-residual test after the cycle's steps
+        // (steps past the first one at or below tol are dropped from the
+        // combine, as the host loop would not have run them), the true
+        // residual test at a restart, and a zero initial residual.  The
+        // kernels of a cycle after the stop return at once.
+        HIPCHK(launch_gmres_start(c->nrm, st, c->stream));
+        const bool use_graph = N <= kDeepLookahead && c->use_graphs && !c->prof;
+        for (long cyc = 0; (1 + cyc * long(m)) / m < maxit; ++cyc) {
+            if (cyc > 0) {
+                HIPCHK(launch_gmres_gate(st, c->stream));
+                CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part, skip));
+                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, c->grid.blocks, 1, c->nrm, c->stream, nullptr, skip));
+                HIPCHK(launch_gmres_restart(c->nrm, tol, st, c->stream));
+            }
+            CHK(enqueue_cycle(use_graph));
+            HIPCHK(launch_gmres_lsq(c->gs, kGsRow, m, tol, st, c->stream));
+            TIMED(HELM3D_PROF_COMBINE, launch_combine_dev<R>(V, ld, m, st->coef, &st->kept, x, N, nblk, c->stream, skip));
+        }
+        // no host copy of the outcome: the caller uses x only
+        res->iters = -1;
+        res->cycles = -1;
+        res->relres = res->quasi = std::nan("");
+        res->converged = false;
+        return HELM3D_OK;
+    }
+
     // Without P and with no result wanted, ||r0|| is read with the first
     // step's row (one host round trip less): the steps need it only on the
     // device, and with r0 = 0 they are discarded.
@@ -253,38 +336,6 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         sig[0] = defer_norm ? 0.0 : 1.0 / beta;
         int k = 1;
         if (!P) {
-            // enqueue Arnoldi step j: slot(j) = A slot(j-1)/||slot(j-1)||, dots into
-            // row j, update with device scales, ||slot(j)||^2 into row j and nrm[j]
-            auto enqueue_on = [&](int j, hipStream_t s, bool record) -> int {
-                double* row = c->gs + size_t(j) * kGsRow;
-                StencilLaunch<R> L{};
-                L.op = op;
-                L.x = slot(j - 1);
-                L.y = slot(j);
-                L.alpha = R(1);
-                L.beta = R(0);
-                L.anrm = c->nrm + (j - 1);
-                L.sink = static_cast<cpx<R>*>(c->sink);
-                TIMED(HELM3D_PROF_STENCIL, launch_stencil<R>(variant, L, c->grid, s));
-                // reduced dots into row (the update's coefficients) and the
-                // pinned ring row, ||slot(j)||^2 into nrm[j] and the ring row
-                double* hrow = c->ring_dev + size_t(j) * kGsRow;
-                if (nblk <= kInBlockDots) {
-                    // short dots columns: summed by every block of the update
-                    // pass (partials one column up, clear of the norm partials)
-                    TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part + nblk, nblk, s));
-                    TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, nullptr, CoefPack{}, slot(j), N, c->part,
-                                                               nblk, s, c->nrm, c->part + nblk, hrow));
-                } else {
-                    TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, s));
-                    TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, s, hrow));
-                    TIMED(HELM3D_PROF_UPDATE,
-                          launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk, s, c->nrm));
-                }
-                TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, s, hrow + 2 * j));
-                if (record) HIPCHK(hipEventRecord(c->ring_ev[j], s));
-                return HELM3D_OK;
-            };
             auto enqueue = [&](int j) { return enqueue_on(j, c->stream, true); };
             // may a step follow once `done` steps have been taken (iteration cap)
             auto may_continue = [&](long done) { return iter_cap <= 0 || done < iter_cap; };
@@ -295,10 +346,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             // after its last step
             const bool graph = depth == m && c->use_graphs && !c->prof && may_continue(inner + m - 1);
             if (graph) {
-                CHK(cycle_graph<R>(c, variant, m, V, ld, nblk, [&](hipStream_t s) -> int {
-                    for (int j = 1; j <= m; ++j) CHK(enqueue_on(j, s, false));
-                    return HELM3D_OK;
-                }));
+                CHK(enqueue_cycle(true));
                 HIPCHK(hipEventRecord(c->ring_ev[m], c->stream));
                 next = m + 1;
             }
@@ -747,12 +795,14 @@ int helm3d_set_model(helm3d_ctx* c, const int n[3], const double h[3], const int
     return HELM3D_OK;
 }
 
-int helm3d_set_launch(helm3d_ctx* c, int kchunk, int tile_rows, int prefetch)
+int helm3d_set_launch(helm3d_ctx* c, int kchunk, int tile_rows, int prefetch, int rows_per_thread)
 {
     CHK(check_model(c));
-    if (!(tile_rows == 4 || tile_rows == 8) || prefetch < 1 || prefetch > 4 || kchunk < 0)
-        return fail(HELM3D_ERR_ARG, "tile_rows in {4,8}, prefetch in 1..4, kchunk >= 0");
-    c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kchunk, tile_rows, prefetch);
+    if (!(tile_rows == 4 || tile_rows == 8) || prefetch < 1 || prefetch > 4 || kchunk < 0 ||
+        !(rows_per_thread == 1 || (rows_per_thread == 2 && tile_rows == 8)))
+        return fail(HELM3D_ERR_ARG,
+                    "tile_rows in {4,8}, prefetch in 1..4, kchunk >= 0, rows_per_thread 1 (or 2 with 8 rows)");
+    c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kchunk, tile_rows, prefetch, rows_per_thread);
     CHK(ensure_work(c));
     c->inner_graph_iters = -1;
     return HELM3D_OK;
@@ -770,18 +820,19 @@ int helm3d_set_option(helm3d_ctx* c, int option, long long value)
     }
 }
 
-int helm3d_get_launch(helm3d_ctx* c, int* kchunk, int* tile_rows, int* prefetch, int* blocks)
+int helm3d_get_launch(helm3d_ctx* c, int* kchunk, int* tile_rows, int* prefetch, int* rows_per_thread, int* blocks)
 {
     CHK(check_model(c));
     if (kchunk) *kchunk = c->grid.kchunk;
     if (tile_rows) *tile_rows = c->grid.ty;
     if (prefetch) *prefetch = c->grid.pf;
+    if (rows_per_thread) *rows_per_thread = c->grid.ry;
     if (blocks) *blocks = c->grid.blocks;
     return HELM3D_OK;
 }
 
 int helm3d_autotune(helm3d_ctx* c, int mode, int precision, int reps, int* kchunk, int* tile_rows, int* prefetch,
-                    double* best_ms)
+                    int* rows_per_thread, double* best_ms)
 {
     CHK(check_model(c));
     if (precision != HELM3D_C128 && precision != HELM3D_C64) return fail(HELM3D_ERR_ARG, "bad precision");
@@ -806,10 +857,14 @@ int helm3d_autotune(helm3d_ctx* c, int mode, int precision, int reps, int* kchun
         const int kc = (c->n[2] + d - 1) / d;
         if (kc >= 8 || d == 1) chunks.push_back(kc);
     }
-    for (int ty : {4, 8}) {
+    // (tile rows, rows per thread): 64 x 4 and 64 x 8 tiles, the latter also
+    // with two rows per thread
+    const int shapes[3][2] = {{4, 1}, {8, 1}, {8, 2}};
+    for (const auto& sh : shapes) {
+        const int ty = sh[0], ry = sh[1];
         for (int pf : {1, 2, 3, 4}) {
             for (int kc : chunks) {
-                c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kc, ty, pf);
+                c->grid = stencil_grid<double>(c->n[0], c->n[1], c->n[2], kc, ty, pf, ry);
                 CHK(ensure_work(c));
                 float ms = 0.f;
                 auto run = [&]() -> int {
@@ -846,6 +901,7 @@ int helm3d_autotune(helm3d_ctx* c, int mode, int precision, int reps, int* kchun
     if (kchunk) *kchunk = best.kchunk;
     if (tile_rows) *tile_rows = best.ty;
     if (prefetch) *prefetch = best.pf;
+    if (rows_per_thread) *rows_per_thread = best.ry;
     if (best_ms) *best_ms = best_t;
     return HELM3D_OK;
 }

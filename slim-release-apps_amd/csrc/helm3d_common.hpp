@@ -31,10 +31,13 @@ template <typename R>
 __host__ __device__ inline cpx<R> operator-(cpx<R> a) { return {-a.re, -a.im}; }
 template <typename R>
 __host__ __device__ inline cpx<R> operator*(R s, cpx<R> a) { return {s * a.re, s * a.im}; }
+// The fused forms are written out so that every kernel instantiation rounds
+// the same way (a * b - c * d left to the compiler may be contracted with
+// either product, differently in different instantiations).
 template <typename R>
 __host__ __device__ inline cpx<R> cmul(cpx<R> a, cpx<R> b)
 {
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return {R(fma(a.re, b.re, -(a.im * b.im))), R(fma(a.re, b.im, a.im * b.re))};
 }
 // acc + a*b
 template <typename R>

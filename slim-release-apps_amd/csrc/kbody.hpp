@@ -169,25 +169,29 @@ __device__ __forceinline__ void update_body(const cpx<R>* __restrict__ V, long l
     if (threadIdx.x == 0) part[b] = s2[0];
 }
 
-// x <- x + sum_{c<K} a_c v_c (the solution update of FGMRES.m:115-119).
+// x <- x + sum_{c<kk} a_c v_c (the solution update of FGMRES.m:115-119);
+// columns kk..K-1 are neither read nor used (kk <= K, uniform).
 template <typename R, int K, bool NT = false>
 __device__ __forceinline__ void combine_body(const cpx<R>* __restrict__ V, long ldv, const double (&pr)[K],
-                                             const double (&pi)[K], cpx<R>* __restrict__ x, long n, int b, int G)
+                                             const double (&pi)[K], cpx<R>* __restrict__ x, long n, int b, int G,
+                                             int kk = K)
 {
     long lo, hi;
     range_even(n, b, G, lo, hi);
     for (long t = lo + threadIdx.x; t < hi; t += KB) {
         cpx<R> v[K];
 #pragma unroll
-        for (int c = 0; c < K; ++c) v[c] = ld_basis<NT>(&V[c * ldv + t]);
+        for (int c = 0; c < K; ++c) v[c] = c < kk ? ld_basis<NT>(&V[c * ldv + t]) : cpx<R>{R(0), R(0)};
         const cpx<R> xt = x[t];
         double re = xt.re, im = xt.im;
 #pragma unroll
         for (int c = 0; c < K; ++c) {
-            re = fma(pr[c], double(v[c].re), re);
-            re = fma(-pi[c], double(v[c].im), re);
-            im = fma(pr[c], double(v[c].im), im);
-            im = fma(pi[c], double(v[c].re), im);
+            if (c < kk) {
+                re = fma(pr[c], double(v[c].re), re);
+                re = fma(-pi[c], double(v[c].im), re);
+                im = fma(pr[c], double(v[c].im), im);
+                im = fma(pi[c], double(v[c].re), im);
+            }
         }
         x[t] = cpx<R>{R(re), R(im)};
     }

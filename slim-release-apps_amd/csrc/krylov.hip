@@ -30,8 +30,9 @@ using kb::KB;
 template <typename R, int K, bool NT>
 __global__ __launch_bounds__(KB) void dots_kernel(const cpx<R>* __restrict__ V, long ldv,
                                                   const cpx<R>* __restrict__ w, long n,
-                                                  double* __restrict__ part)
+                                                  double* __restrict__ part, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     kb::dots_body<R, K, NT>(V, ldv, w, n, part, blockIdx.x, gridDim.x);
 }
 
@@ -49,8 +50,9 @@ __global__ __launch_bounds__(KB) void update_kernel(const cpx<R>* __restrict__ V
                                                     const double* __restrict__ nrm, cpx<R>* __restrict__ w,
                                                     long n, double* __restrict__ part,
                                                     const double* __restrict__ dpart, int dnblk,
-                                                    double* __restrict__ hrow)
+                                                    double* __restrict__ hrow, const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     __shared__ double dsum[2 * K];
     if (dpart) {
         const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -88,11 +90,149 @@ __global__ __launch_bounds__(KB) void combine_kernel(const cpx<R>* __restrict__ 
     kb::combine_body<R, K, NT>(V, ldv, pr, pi, x, n, blockIdx.x, gridDim.x);
 }
 
+// x <- x + sum_{c<kept} a_c v_c with the coefficients in device memory (a:
+// 2K doubles, written by gmres_lsq_kernel, which also writes *kept <= K);
+// the slots of steps past the kept ones are not read, so values a step
+// after a breakdown left there (0 * inf) do not reach x.
+template <typename R, int K, bool NT>
+__global__ __launch_bounds__(KB) void combine_dev_kernel(const cpx<R>* __restrict__ V, long ldv,
+                                                         const double* __restrict__ a, const int* __restrict__ kept,
+                                                         cpx<R>* __restrict__ x, long n, const int* __restrict__ skip)
+{
+    if (skip && *skip) return;
+    const int kk = min(*kept, K);
+    double pr[K], pi[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        pr[c] = a[2 * c];
+        pi[c] = a[2 * c + 1];
+    }
+    kb::combine_body<R, K, NT>(V, ldv, pr, pi, x, n, blockIdx.x, gridDim.x, kk);
+}
+
+// ---- result-free restarted GMRES driven from the device ----------------------
+// The host enqueues every cycle a solve may need; the Hessenberg least
+// squares of FGMRES.m:109-111 runs in gmres_lsq_kernel (one thread), and the
+// stop decisions of the host loop (quasi-residual per step, true residual per
+// restart) set st->stop; the kernels of later cycles read st->skip and return.
+
+struct Cx {
+    double re, im;
+};
+__device__ inline Cx cx_add(Cx a, Cx b) { return Cx{a.re + b.re, a.im + b.im}; }
+__device__ inline Cx cx_sub(Cx a, Cx b) { return Cx{a.re - b.re, a.im - b.im}; }
+__device__ inline Cx cx_mul(Cx a, Cx b) { return Cx{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)}; }
+__device__ inline Cx cx_conj(Cx a) { return Cx{a.re, -a.im}; }
+__device__ inline double cx_norm(Cx a) { return fma(a.re, a.re, a.im * a.im); }
+__device__ inline Cx cx_scale(Cx a, double s) { return Cx{a.re * s, a.im * s}; }
+__device__ inline Cx cx_div(Cx a, Cx b)
+{
+    const double d = cx_norm(b);
+    return Cx{fma(a.re, b.re, a.im * b.im) / d, fma(a.im, b.re, -(a.re * b.im)) / d};
+}
+
+// Start of a device-driven solve: ||r0|| from nrm[0]; a zero residual stops
+// before the first step (x is already the answer).
+__global__ void gmres_start_kernel(const double* __restrict__ nrm, GmresDevState* st)
+{
+    const double b = sqrt(nrm[0]);
+    st->normr0 = b;
+    st->beta = b;
+    st->hlast = 1.0;
+    st->stop = b == 0.0 ? 1 : 0;
+    st->skip = st->stop;
+}
+
+// Start of a later cycle (restart): stop if an earlier step or restart
+// test stopped the solve, else read ||r|| of the new residual from nrm[0]
+// and stop on the true residual (FGMRES.m:120-124).
+__global__ void gmres_gate_kernel(GmresDevState* st) { st->skip = st->stop; }
+__global__ void gmres_restart_kernel(const double* __restrict__ nrm, double tol, GmresDevState* st)
+{
+    if (st->skip) return;
+    const double rn = sqrt(nrm[0]);
+    if (rn / st->normr0 < tol) {
+        st->stop = 1;
+        st->skip = 1;
+        return;
+    }
+    st->beta = rn;
+}
+
+// The least squares of one cycle of m steps (FGMRES.m:109-111): rows[j]
+// (stride ldr) hold the dots d_0..d_{j-1} of step j and ||w_j||^2; the
+// Hessenberg column of step j is sig_q d_q (q < j) over ||w_j||, with the
+// basis slot scales sig_q = 1/||v_q||.  The columns are reduced as they come
+// by Givens rotations (the triangular factor and the rotated right-hand side
+// in LDS), so the quasi-residual of step k is |g_k| / ||r0|| -- the residual
+// of the k-step least squares -- after O(k) work.  Steps after the first one
+// at or below tol are dropped (the host loop would not have run them); coef
+// receives a_q = y_q sig_q for the kept steps, zeros after.  One thread.
+__global__ void gmres_lsq_kernel(const double* __restrict__ rows, int ldr, int m, double tol, GmresDevState* st)
+{
+    constexpr int ld = kMaxBasis + 1;
+    __shared__ Cx T[ld * kMaxBasis];  // triangular factor, column-major
+    __shared__ Cx cs[kMaxBasis], sn[kMaxBasis], g[kMaxBasis + 1], y[kMaxBasis];
+    __shared__ double sig[kMaxBasis + 1];
+    if (threadIdx.x != 0 || st->skip) return;
+    const double normr0 = st->normr0;
+    sig[0] = 1.0 / st->beta;
+    g[0] = Cx{st->beta, 0.0};
+    int kept = m;
+    double hlast = st->hlast;
+    for (int k = 1; k <= m; ++k) {
+        const double* h = rows + long(k) * ldr;
+        Cx* col = T + (k - 1) * ld;
+        for (int q = 0; q < k; ++q) col[q] = cx_scale(Cx{h[2 * q], h[2 * q + 1]}, sig[q]);
+        const double hn = sqrt(h[2 * k]);
+        sig[k] = 1.0 / hn;
+        for (int q = 0; q + 1 < k; ++q) {
+            const Cx u = col[q], v = col[q + 1];
+            col[q] = cx_add(cx_mul(cx_conj(cs[q]), u), cx_mul(cx_conj(sn[q]), v));
+            col[q + 1] = cx_sub(cx_mul(cs[q], v), cx_mul(sn[q], u));
+        }
+        // the rotation that zeroes ||w_k|| below the diagonal
+        const Cx a = col[k - 1];
+        const double den = sqrt(cx_norm(a) + hn * hn);
+        Cx c{1.0, 0.0}, sv{0.0, 0.0};
+        if (den != 0.0) {
+            c = cx_scale(a, 1.0 / den);
+            sv = Cx{hn / den, 0.0};
+        }
+        cs[k - 1] = c;
+        sn[k - 1] = sv;
+        col[k - 1] = Cx{den, 0.0};
+        const Cx gk = g[k - 1];
+        g[k - 1] = cx_mul(cx_conj(c), gk);
+        g[k] = cx_scale(cx_mul(sv, gk), -1.0);
+        hlast = sqrt(cx_norm(g[k])) / normr0;
+        if (hlast <= tol) {
+            kept = k;
+            break;
+        }
+    }
+    for (int r = kept - 1; r >= 0; --r) {
+        Cx acc = g[r];
+        for (int c = r + 1; c < kept; ++c) acc = cx_sub(acc, cx_mul(T[r + c * ld], y[c]));
+        y[r] = cx_div(acc, T[r + r * ld]);
+    }
+    for (int q = 0; q < kMaxBasis; ++q) {
+        const Cx a = q < kept ? cx_scale(y[q], sig[q]) : Cx{0.0, 0.0};
+        st->coef[2 * q] = a.re;
+        st->coef[2 * q + 1] = a.im;
+    }
+    st->hlast = hlast;
+    st->kept = kept;
+    if (hlast <= tol) st->stop = 1;
+}
+
 // out[q] = sum_b part[q*nblk + b] in block order; one block per column q
 // (out2, when given, receives the same values).
 __global__ __launch_bounds__(KB) void reduce_kernel(const double* __restrict__ part, int nblk,
-                                                    double* __restrict__ out, double* __restrict__ out2)
+                                                    double* __restrict__ out, double* __restrict__ out2,
+                                                    const int* __restrict__ skip)
 {
+    if (skip && *skip) return;
     const double s = kb::reduce_body(part, nblk, blockIdx.x);
     if (threadIdx.x == 0) {
         out[blockIdx.x] = s;
@@ -180,14 +320,16 @@ bool stream_basis(int k, long n)
 
 template <typename R>
 hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n, double* part, int nblk,
-                       hipStream_t s)
+                       hipStream_t s, const int* skip)
 {
     if (k < 1) return hipErrorInvalidValue;
     const bool nt = stream_basis<R>(k, n);
     return dispatch_k<1>(k, [&](auto kc) {
         constexpr int K = decltype(kc)::value;
-        if (nt) hipLaunchKernelGGL((dots_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part);
-        else hipLaunchKernelGGL((dots_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part);
+        if (nt)
+            hipLaunchKernelGGL((dots_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part, skip);
+        else
+            hipLaunchKernelGGL((dots_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, w, n, part, skip);
         return hipGetLastError();
     });
 }
@@ -195,7 +337,7 @@ hipError_t launch_dots(const cpx<R>* V, long ldv, int k, const cpx<R>* w, long n
 template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
                          long n, double* part, int nblk, hipStream_t s, const double* nrm, const double* dpart,
-                         double* hrow)
+                         double* hrow, const int* skip)
 {
     if (k < 1) return hipErrorInvalidValue;
     const bool nt = stream_basis<R>(k, n);
@@ -203,12 +345,52 @@ hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, cons
         constexpr int K = decltype(kc)::value;
         if (nt)
             hipLaunchKernelGGL((update_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, nrm, w, n,
-                               part, dpart, nblk, hrow);
+                               part, dpart, nblk, hrow, skip);
         else
             hipLaunchKernelGGL((update_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, d, scl, nrm, w, n,
-                               part, dpart, nblk, hrow);
+                               part, dpart, nblk, hrow, skip);
         return hipGetLastError();
     });
+}
+
+template <typename R>
+hipError_t launch_combine_dev(const cpx<R>* V, long ldv, int k, const double* a, const int* kept, cpx<R>* x,
+                              long n, int nblk, hipStream_t s, const int* skip)
+{
+    if (k <= 0) return hipSuccess;
+    const bool nt = stream_basis<R>(k, n);
+    return dispatch_k<1>(k, [&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        if (nt)
+            hipLaunchKernelGGL((combine_dev_kernel<R, K, true>), dim3(nblk), dim3(KB), 0, s, V, ldv, a, kept, x, n,
+                               skip);
+        else
+            hipLaunchKernelGGL((combine_dev_kernel<R, K, false>), dim3(nblk), dim3(KB), 0, s, V, ldv, a, kept, x, n,
+                               skip);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_gmres_start(const double* nrm, GmresDevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_start_kernel, dim3(1), dim3(1), 0, s, nrm, st);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_gate(GmresDevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_gate_kernel, dim3(1), dim3(1), 0, s, st);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_restart(const double* nrm, double tol, GmresDevState* st, hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_restart_kernel, dim3(1), dim3(1), 0, s, nrm, tol, st);
+    return hipGetLastError();
+}
+hipError_t launch_gmres_lsq(const double* rows, int ldr, int m, double tol, GmresDevState* st, hipStream_t s)
+{
+    if (m < 1 || m >= kMaxBasis) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gmres_lsq_kernel, dim3(1), dim3(1), 0, s, rows, ldr, m, tol, st);
+    return hipGetLastError();
 }
 
 template <typename R>
@@ -225,9 +407,10 @@ hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, c
     });
 }
 
-hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s, double* out2)
+hipError_t launch_reduce(const double* part, int nblk, int nq, double* out, hipStream_t s, double* out2,
+                         const int* skip)
 {
-    hipLaunchKernelGGL(reduce_kernel, dim3(nq), dim3(KB), 0, s, part, nblk, out, out2);
+    hipLaunchKernelGGL(reduce_kernel, dim3(nq), dim3(KB), 0, s, part, nblk, out, out2, skip);
     return hipGetLastError();
 }
 
@@ -265,10 +448,12 @@ hipError_t launch_norm2(const cpx<R>* x, long n, double* part, int nblk, hipStre
 
 #define HELM3D_KRYLOV_INST(R)                                                                               \
     template hipError_t launch_dots<R>(const cpx<R>*, long, int, const cpx<R>*, long, double*, int,        \
-                                       hipStream_t);                                                        \
+                                       hipStream_t, const int*);                                            \
     template hipError_t launch_update<R>(const cpx<R>*, long, int, const double*, const CoefPack&, cpx<R>*,  \
                                          long, double*, int, hipStream_t, const double*, const double*,     \
-                                         double*);                                                          \
+                                         double*, const int*);                                              \
+    template hipError_t launch_combine_dev<R>(const cpx<R>*, long, int, const double*, const int*, cpx<R>*, \
+                                              long, int, hipStream_t, const int*);                          \
     template hipError_t launch_combine<R>(const cpx<R>*, long, int, const CoefPack&, cpx<R>*, long, int,    \
                                           hipStream_t);                                                     \
     template hipError_t launch_norm2<R>(const cpx<R>*, long, double*, int, hipStream_t);

@@ -140,10 +140,10 @@ _SIGS = [
                                        ctypes.POINTER(SolveInfo)]),
     ("helm3d_solve_mg_mixed", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts), _ip,
                                              ctypes.POINTER(SolveInfo)]),
-    ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("helm3d_set_launch", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("helm3d_set_option", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong]),
-    ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
-    ("helm3d_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _dp]),
+    ("helm3d_get_launch", ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
+    ("helm3d_autotune", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip, _dp]),
     ("helm3d_profile", ctypes.c_int, [_vp, ctypes.c_int]),
     ("helm3d_profile_read", ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 ]

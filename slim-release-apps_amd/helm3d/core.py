@@ -272,27 +272,30 @@ class Helm3d:
             check(lib().helm3d_set_option(self._ctx, _lib.HELM3D_OPT_GRAPHS, int(enable)))
 
     # -- launch shape ---------------------------------------------------------
-    def set_launch(self, kchunk: int = 0, tile_rows: int = 4, prefetch: int = 1):
+    def set_launch(self, kchunk: int = 0, tile_rows: int = 4, prefetch: int = 1, rows_per_thread: int = 1):
         """Stencil launch shape (results are identical for every shape)."""
         with torch.cuda.device(self.device):
-            check(lib().helm3d_set_launch(self._ctx, int(kchunk), int(tile_rows), int(prefetch)))
+            check(lib().helm3d_set_launch(self._ctx, int(kchunk), int(tile_rows), int(prefetch),
+                                          int(rows_per_thread)))
 
     def autotune(self, mode: int = 1, precision: str = "c128", reps: int = 10) -> dict:
         """Pick the fastest stencil launch shape for this grid (helm3d_autotune)."""
         prec = {"c128": _lib.HELM3D_C128, "c64": _lib.HELM3D_C64}[precision]
-        v = [ctypes.c_int(0) for _ in range(3)]
+        v = [ctypes.c_int(0) for _ in range(4)]
         ms = ctypes.c_double(0)
         with torch.cuda.device(self.device):
             self._bind()
             check(lib().helm3d_autotune(self._ctx, int(mode), prec, int(reps), *[ctypes.byref(a) for a in v],
                                         ctypes.byref(ms)))
-        return dict(kchunk=v[0].value, tile_rows=v[1].value, prefetch=v[2].value, ms=ms.value)
+        return dict(kchunk=v[0].value, tile_rows=v[1].value, prefetch=v[2].value, rows_per_thread=v[3].value,
+                    ms=ms.value)
 
     def get_launch(self) -> dict:
-        v = [ctypes.c_int(0) for _ in range(4)]
+        v = [ctypes.c_int(0) for _ in range(5)]
         with torch.cuda.device(self.device):
             check(lib().helm3d_get_launch(self._ctx, *[ctypes.byref(a) for a in v]))
-        return dict(kchunk=v[0].value, tile_rows=v[1].value, prefetch=v[2].value, blocks=v[3].value)
+        return dict(kchunk=v[0].value, tile_rows=v[1].value, prefetch=v[2].value, rows_per_thread=v[3].value,
+                    blocks=v[4].value)
 
     # -- in-library kernel timing -------------------------------------------
     def profile(self, enable: bool = True):

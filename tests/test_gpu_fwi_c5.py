@@ -1,6 +1,6 @@
 """Gradient check of the FWI objective at full size: 256x256x128 + 20-point
 PML at 12.5 m, 16 sources, 64x64 receivers, 4 and 6 Hz, squared slowness,
-every solve to 1e-10 (FGMRES + 4-level complex64 multigrid V-cycle, eight
+every solve to 1e-11 (FGMRES + 4-level complex64 multigrid V-cycle, eight
 hierarchies solving in turn).
 
 Central differences of the device objective against the device gradient on
@@ -13,7 +13,15 @@ one forward model, observed data and gradient per setting):
     are used;
   * cut_pml = False: the gradient is folded back with the extension adjoint
     and is the full derivative; unrestricted random directions are used.
-Bar: relative difference <= 1e-4 per direction."""
+Bar: relative difference <= 1e-4 per direction.  The central difference
+divides the objective's solver noise by the step, so that noise, not the
+gradient, decides directions whose derivative is small: with solves to
+1e-10 and a step of 1e-4 the absolute difference stayed near 0.02-0.04 on
+every direction, i.e. 1.04e-4 relative on one whose derivative was 20x below
+the typical one.  The solves here go to 1e-11 and the step is 3e-4 of a
+direction scaled to max(m0), which puts that noise near 1e-3 absolute;
+central differences are exact for the quadratic part of the objective and
+the next term is O(eps^2)."""
 import numpy as np
 import pytest
 
@@ -34,7 +42,7 @@ def _state(cut_pml):
         gc.collect()
         torch.cuda.empty_cache()
         model, m_true, m0 = fwi_c5()
-        P = FWIProblem(model, FWIOpts(pml=20, pml_max=20, tol=1e-10, solver="mg", mg_levels=4,
+        P = FWIProblem(model, FWIOpts(pml=20, pml_max=20, tol=1e-11, solver="mg", mg_levels=4,
                                       mg_precision="mixed", mg_workers=8, mg_k=(1, 2, 3, 5), cut_pml=cut_pml),
                        device=torch.device("cuda:0"))
         print(f"C5 gradient check, cut_pml={cut_pml}: forward data and gradient", flush=True)
@@ -60,7 +68,7 @@ def _release_device_state():
 def test_c5_gradient_vs_central_differences(cut_pml, chunk):
     P, D, m0, g, n = _state(cut_pml)
     rng = np.random.default_rng(20261205 + 1000 * int(cut_pml) + chunk)
-    eps = 1e-4
+    eps = 3e-4
     for it in range(CHUNK):
         dm = rng.standard_normal(n[::-1]).transpose(2, 1, 0)  # (nx, ny, nz)
         if cut_pml:

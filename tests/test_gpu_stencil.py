@@ -114,22 +114,61 @@ def test_device_deriv():
     assert relerr(ya, O.hmvp("adj_deriv", dwn, h, n, np.array(npml), x)) <= 1e-12
 
 
-@pytest.mark.parametrize("shape", [(0, 4, 1), (35, 4, 2), (70, 8, 1), (17, 8, 2), (23, 4, 3), (9, 8, 4), (70, 4, 4)])
+SHAPES = [(0, 4, 1, 1), (35, 4, 2, 1), (70, 8, 1, 1), (17, 8, 2, 1), (23, 4, 3, 1), (9, 8, 4, 1), (70, 4, 4, 1),
+          (0, 8, 1, 2), (70, 8, 2, 2), (17, 8, 3, 2), (9, 8, 4, 2)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
 def test_launch_shapes_identical(shape):
-    """Every launch shape gives bit-identical results (interior and PML waves)."""
+    """Every launch shape (z chunk, tile rows, prefetch, rows per thread) gives
+    bit-identical results, interior and PML waves: forward and adjoint with
+    real and complex wn, the derivative operator, complex64, and the gradient
+    epilogue."""
     import torch
     from helm3d import Helm3d
     n, npml, h = CASES[5]
     wn, x = _inputs(n, 55, False)
+    wnc, _ = _inputs(n, 56, True)
     dev = torch.device("cuda:0")
     H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).to(dev))
+    Hc = Helm3d(n, h, np.array(npml), torch.from_numpy(wnc).to(dev))
     xt = torch.from_numpy(x).to(dev)
-    y0 = H.apply(xt).clone()
-    a0 = H.apply(xt, adjoint=True).clone()
+    dwn = torch.from_numpy(np.random.default_rng(57).uniform(0.5, 2.0, wn.size)).to(dev)
+    U = torch.flip(xt, [0]).contiguous()
+
+    def products():
+        g = torch.zeros(wn.size, dtype=torch.float64, device=dev)
+        H.grad_accum(U, xt, g, scale=0.7, grid="pml", dwn=dwn)
+        gp = torch.ones(int(np.prod(H.n_phys)), dtype=torch.float64, device=dev)
+        H.grad_accum(U, xt, gp, scale=1.3, grid="phys")
+        return [H.apply(xt), H.apply(xt, adjoint=True), Hc.apply(xt), Hc.apply(xt, adjoint=True),
+                H.apply_deriv(dwn, xt), H.apply_deriv(dwn, xt, adjoint=True), H.apply(xt.to(torch.complex64)),
+                H.apply(xt.to(torch.complex64), adjoint=True), g, gp]
+
+    ref = [t.clone() for t in products()]
     H.set_launch(*shape)
-    assert torch.equal(H.apply(xt), y0)
-    assert torch.equal(H.apply(xt, adjoint=True), a0)
-    assert relerr(y0.cpu().numpy(), O.hmvp("forw", wn, h, n, np.array(npml), x)) <= 1e-12
+    Hc.set_launch(*shape)
+    for k, (a, b) in enumerate(zip(products(), ref)):
+        assert torch.equal(a, b), (k, float((a - b).abs().max() / b.abs().max()))
+    assert relerr(ref[0].cpu().numpy(), O.hmvp("forw", wn, h, n, np.array(npml), x)) <= 1e-12
+
+
+def test_solve_with_two_rows_per_thread():
+    """A solve on the 64 x 8, two-rows-per-thread shape: the products are
+    bit-identical and only the in-block order of the norm partials changes,
+    so the iteration count is the same and x agrees to rounding."""
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_small, extend, point_source, vec, wavenumber
+    cfg = config_small()
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    b = torch.from_numpy(point_source(cfg, cfg.sources[0])).cuda()
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).cuda())
+    x1, i1 = H.solve(b, tol=1e-8)
+    H.set_launch(0, 8, 2, 2)
+    x2, i2 = H.solve(b, tol=1e-8)
+    assert i1.converged and i2.converged and i1.iters == i2.iters
+    assert float((x1 - x2).norm() / x1.norm()) < 1e-9
 
 
 def test_c2_full_size_matvec_parity():
@@ -166,8 +205,10 @@ def test_autotune_keeps_results():
         r = H.autotune(mode=mode, precision=prec, reps=3)
         print("autotune", mode, prec, r, flush=True)
         assert r["tile_rows"] in (4, 8) and r["prefetch"] in (1, 2, 3, 4) and 1 <= r["kchunk"] <= n[2] and r["ms"] > 0
+        assert r["rows_per_thread"] == 1 or (r["rows_per_thread"] == 2 and r["tile_rows"] == 8)
         L = H.get_launch()
-        assert (L["kchunk"], L["tile_rows"], L["prefetch"]) == (r["kchunk"], r["tile_rows"], r["prefetch"])
+        keys = ("kchunk", "tile_rows", "prefetch", "rows_per_thread")
+        assert tuple(L[q] for q in keys) == tuple(r[q] for q in keys)
         assert torch.equal(H.apply(xt), y0)
         assert torch.equal(H.apply(xt, adjoint=True), a0)
         assert torch.equal(H.apply(xt.to(torch.complex64)), s0)
