@@ -135,6 +135,23 @@ int helm3d_apply(helm3d_ctx *ctx, int variant, int precision, const void *x_dev,
 int helm3d_apply_deriv(helm3d_ctx *ctx, int adjoint, int precision, const void *dwn_dev,
                        const void *x_dev, void *y_dev);
 
+/* y = H x (variant HELM3D_FORW) or H^H x (HELM3D_ADJ) in complex128 for the
+ * operator given by n (points per axis, PML included), h, npml[6] and wn on
+ * the device (N doubles, or N interleaved complex doubles when wn_complex),
+ * without any device allocation: the variant of helm3d_apply for callers that
+ * cannot call hipMalloc (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference.  The kernels' scratch (the PML tables and
+ * the stencil's write-only sink) is a static device array of the library,
+ * 1.4 MiB per GPU, held until the process ends; each axis may hold at most
+ * HELM3D_NOALLOC_MAX_AXIS points (HELM3D_ERR_ARG beyond).  Work is issued on
+ * `stream` (a hipStream_t, NULL: the default stream) and is asynchronous.
+ * Calls that overlap on one GPU race on that array; order them (one stream,
+ * or events). */
+#define HELM3D_NOALLOC_MAX_AXIS 4094
+int helm3d_apply_noalloc(int variant, const int n[3], const double h[3], const int npml[6],
+                         const void *wn_dev, int wn_complex, const void *x_dev, void *y_dev,
+                         void *stream);
+
 /* Stencil launch shape: z planes per block (0: automatic), column-tile rows
  * (4 or 8), planes of loads in flight (1 to 4; the batched kernels use at
  * most 2) and rows per thread (1, or 2 with 8 tile rows: 256 threads per

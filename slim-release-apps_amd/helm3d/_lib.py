@@ -40,6 +40,8 @@ HELM3D_BATCH_PLAIN = 0
 HELM3D_BATCH_MIXED = 1
 HELM3D_BATCH_MAX_ITEMS = 1024  # items per helm3d_batch_solve call
 
+HELM3D_NOALLOC_MAX_AXIS = 4094  # points per axis of helm3d_apply_noalloc
+
 HELM3D_GRID_PHYS = 0
 HELM3D_GRID_PML = 1
 
@@ -99,6 +101,7 @@ _SIGS = [
     ("helm3d_set_model", ctypes.c_int, [_vp, _ip, _dp, _ip, _vp, ctypes.c_int]),
     ("helm3d_numel", ctypes.c_longlong, [_vp]),
     ("helm3d_apply", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("helm3d_apply_noalloc", ctypes.c_int, [ctypes.c_int, _ip, _dp, _ip, _vp, ctypes.c_int, _vp, _vp, _vp]),
     ("helm3d_apply_deriv", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
     ("helm3d_solve_opts_default", None, [ctypes.POINTER(SolveOpts)]),
     ("helm3d_solve", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.POINTER(SolveOpts),
