@@ -36,8 +36,9 @@ struct MgLevel {
 
 // Transfer tables of one axis: LinInterp1D(linspace(0,1,nc), linspace(0,1,nf)).
 struct AxisTab {
-    std::vector<int> I, rp, fi;
-    std::vector<double> wa, wb, fw;
+    std::vector<int> I, rp, fi, pfi;
+    std::vector<double> wa, wb, fw, pfw;
+    int pw = 0;
 };
 
 AxisTab axis_tab(int nf, int nc)
@@ -78,6 +79,17 @@ AxisTab axis_tab(int nf, int nc)
         }
         t.rp.push_back(int(t.fi.size()));
     }
+    // padded lists: every coarse index gets pw entries
+    for (int c = 0; c < nc; ++c) t.pw = std::max(t.pw, t.rp[c + 1] - t.rp[c]);
+    t.pfi.assign(size_t(nc) * t.pw, 0);
+    t.pfw.assign(size_t(nc) * t.pw, 0.0);
+    for (int c = 0; c < nc; ++c)
+        for (int e = 0; e < t.pw; ++e) {
+            const int q = t.rp[c] + e;
+            const bool real = q < t.rp[c + 1];
+            t.pfi[size_t(c) * t.pw + e] = real ? t.fi[q] : (t.rp[c] < t.rp[c + 1] ? t.fi[t.rp[c]] : 0);
+            t.pfw[size_t(c) * t.pw + e] = real ? t.fw[q] : 0.0;
+        }
     return t;
 }
 
@@ -301,8 +313,8 @@ int helm3d_mg_create(int nlevels, helm3d_ctx* const* A, helm3d_ctx* const* S, he
         for (int a = 0; a < 3; ++a) {
             tabs.push_back(axis_tab(A[l]->n[a], A[l + 1]->n[a]));
             const AxisTab& t = tabs.back();
-            ints += t.I.size() + t.rp.size() + t.fi.size();
-            dbls += t.wa.size() + t.wb.size() + t.fw.size();
+            ints += t.I.size() + t.rp.size() + t.fi.size() + t.pfi.size();
+            dbls += t.wa.size() + t.wb.size() + t.fw.size() + t.pfw.size();
         }
     auto cleanup = [&](int rc) {
         delete mg;
@@ -328,7 +340,7 @@ int helm3d_mg_create(int nlevels, helm3d_ctx* const* A, helm3d_ctx* const* S, he
     hd.reserve(dbls);
     hi.reserve(ints);
     struct Off {
-        size_t I, wa, wb, rp, fi, fw;
+        size_t I, wa, wb, rp, fi, fw, pfi, pfw;
     };
     std::vector<Off> offs;
     for (const AxisTab& t : tabs) {
@@ -345,6 +357,10 @@ int helm3d_mg_create(int nlevels, helm3d_ctx* const* A, helm3d_ctx* const* S, he
         hi.insert(hi.end(), t.rp.begin(), t.rp.end());
         o.fi = hi.size();
         hi.insert(hi.end(), t.fi.begin(), t.fi.end());
+        o.pfw = hd.size();
+        hd.insert(hd.end(), t.pfw.begin(), t.pfw.end());
+        o.pfi = hi.size();
+        hi.insert(hi.end(), t.pfi.begin(), t.pfi.end());
         offs.push_back(o);
     }
     double* dd = static_cast<double*>(mg->tab_buf);
@@ -358,7 +374,8 @@ int helm3d_mg_create(int nlevels, helm3d_ctx* const* A, helm3d_ctx* const* S, he
             const Off& o = offs[size_t(l) * 3 + a];
             T.nf[a] = A[l]->n[a];
             T.nc[a] = A[l + 1]->n[a];
-            T.ax[a] = MgAxis{di + o.I, dd + o.wa, dd + o.wb, di + o.rp, di + o.fi, dd + o.fw};
+            T.ax[a] = MgAxis{di + o.I,  dd + o.wa,  dd + o.wb,  di + o.rp,
+                             di + o.fi, dd + o.fw, di + o.pfi, dd + o.pfw, tabs[size_t(l) * 3 + a].pw};
         }
     }
     *out = mg;

@@ -16,6 +16,11 @@ struct MgAxis {
     const int* rp;     // [nc+1] row pointers of the transposed lists
     const int* fi;     // fine indices using each coarse index
     const double* fw;  // their weights
+    // the transposed lists padded to pw entries per coarse index (weight 0,
+    // a valid fine index): [nc * pw]
+    const int* pfi;
+    const double* pfw;
+    int pw;
 };
 
 struct MgAxes {
