@@ -468,6 +468,57 @@ def fwi_leg(solver="mg", workers=8):
                        "misfit": f, "grad_norm": float(g.norm())}
 
 
+def kacz_leg(sweeps=5):
+    """The Kaczmarz path (Helm3dkaczswp_mex.c, CGMN.m, CRMN.m): double sweeps
+    of the 240x240x140 benchmark model timed with events (forward and adjoint
+    rows), and CGMN and CRMN solves of the C1 problem (64^3 + 10-pt PML, 5 Hz)
+    to 1e-6.  Algorithmic bytes of a double sweep: each of the 54 colour
+    launches reads and writes x at every point and reads wn (40 B/pt), and
+    each row reads b once per direction (2 x 16 B/pt): 2192 B/pt."""
+    import numpy as np
+    import torch
+    from helm3d import Helm3d
+    from helm3d.models import config_c1, config_c2, extend, point_source, vec, wavenumber
+    dev = torch.device("cuda:0")
+    cfg = config_c2()
+    wn = vec(wavenumber(extend(cfg.v, cfg.npml), cfg.freq))
+    H = Helm3d(cfg.nt, (cfg.h,) * 3, cfg.npml6, torch.from_numpy(wn).to(dev))
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.standard_normal(H.N) + 1j * rng.standard_normal(H.N)).to(dev)
+    b = torch.from_numpy(rng.standard_normal(H.N) + 1j * rng.standard_normal(H.N)).to(dev)
+    alg = 2192.0 * H.N
+    out = {"workload": "double Kaczmarz sweeps (27-colour order, omega 1.5) on the 240x240x140 benchmark model; "
+                       "CGMN and CRMN solves of C1 (64^3 + 10-pt PML, 5 Hz, one impulse) to 1e-6",
+           "alg_bytes_per_double_sweep": alg}
+    for mode, name in ((1, "forward"), (2, "adjoint")):
+        H.kacz_sweep(x, b, mode=mode)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        H.kacz_sweep(x, b, mode=mode, nsweeps=sweeps)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / sweeps
+        out[name] = {"ms_per_double_sweep": ms, "GB/s": alg / (ms * 1e-3) / 1e9,
+                     "frac_of_hbm_peak": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    del H, x, b
+    torch.cuda.empty_cache()
+    c1 = config_c1()
+    wn1 = vec(wavenumber(extend(c1.v, c1.npml), c1.freq))
+    H1 = Helm3d(c1.nt, (c1.h,) * 3, c1.npml6, torch.from_numpy(wn1).to(dev))
+    b1 = torch.from_numpy(point_source(c1, c1.sources[0])).to(dev)
+    for method in ("cgmn", "crmn"):
+        H1.solve_cgmn(b1, tol=1e-6, maxit=20000, method=method)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, info = H1.solve_cgmn(b1, tol=1e-6, maxit=20000, method=method)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[method + "_c1"] = {"rhs_per_s": 1.0 / dt, "seconds": dt, "iters": info.iters,
+                               "relres": info.relres, "converged": info.converged}
+    return out
+
+
 def copy_mode(args):
     """K copies of a 256 MiB buffer: the known-size kernel that calibrates the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -548,6 +599,7 @@ def main():
     ap.add_argument("--no-fwi", action="store_true", help="skip the FWI gradient leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven batched leg")
     ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-preconditioned leg")
+    ap.add_argument("--no-kacz", action="store_true", help="skip the Kaczmarz / CGMN leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default stencil launch shape")
     ap.add_argument("--launch", default=None,
                     help="--mode matvec only: stencil launch shape 'kchunk,tile_rows,prefetch[,rows_per_thread]' "
@@ -561,11 +613,12 @@ def main():
                     help="--mode mg only: one 'levels,precision[,k1,k2,k3,k4]' set (e.g. 4,mixed or "
                          "4,mixed,1,5,3,5) instead of the default list")
     ap.add_argument("--mode", choices=("solve", "matvec", "copy", "mixed", "uneven", "fwi", "mg", "mg-sources",
-                                       "cpu-baseline"),
+                                       "kacz", "cpu-baseline"),
                     default="solve",
                     help="solve: the benchmark; matvec: K FP64 forward stencil launches at the same grid "
                          "(profiling workload); copy: K device copies of 256 MiB (PMC calibration); "
-                         "mixed / uneven / fwi / mg / mg-sources: one of the side legs alone (profiling workloads); "
+                         "mixed / uneven / fwi / mg / mg-sources / kacz: one of the side legs alone (profiling "
+                         "workloads); "
                          "cpu-baseline: the CPU part of the baseline alone (a child process of the default run)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -585,6 +638,9 @@ def main():
 
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode == "kacz":
+        print(json.dumps({"mode": "kacz", **kacz_leg()}))
+        return
     if args.mode in ("mixed", "uneven", "fwi", "mg", "mg-sources"):
         plain_rhs_per_s = None  # no plain solve in this run: ratios are left out
         if args.mode == "mixed":
@@ -730,6 +786,11 @@ def main():
         out["batched_uneven"] = uneven_leg(args.steps / elapsed, args.mixed_restart)
         print(f"[bench] uneven batched leg done: {out['batched_uneven']['rhs_per_s']:.3f} RHS/s", file=sys.stderr,
               flush=True)
+        torch.cuda.empty_cache()
+    if not args.no_kacz:
+        out["kaczmarz"] = kacz_leg()
+        print(f"[bench] kaczmarz leg done: {out['kaczmarz']['forward']['ms_per_double_sweep']:.2f} ms per double "
+              f"sweep", file=sys.stderr, flush=True)
         torch.cuda.empty_cache()
     if not args.no_fwi:
         out["fwi_grads_per_min"], out["fwi"] = fwi_leg()

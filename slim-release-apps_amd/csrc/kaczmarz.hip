@@ -74,77 +74,80 @@ __device__ inline cpx<double> wn_at(const OpDesc<double>& op, long q)
     return cpx<double>{op.wn ? op.wn[q] : op.wn_const, 0.0};
 }
 
+// One thread per row of colour (ci, cj, ck) on a 3-D grid of 64 x 4 blocks
+// (row index x along the lanes, one y row per wave, one z plane per grid
+// layer).  The 27 coefficients and neighbour values are formed once and
+// kept in registers: the 27 loads of a row issue together, and the update
+// writes back x_q + d conj(a_pq) from the values already read (rows of a
+// colour have disjoint supports, so no other row touches them).
 template <bool ADJ>
 __global__ __launch_bounds__(KB) void kacz_color_kernel(OpDesc<double> op, double omega, cpx<double>* __restrict__ x,
                                                          const cpx<double>* __restrict__ b, int ci, int cj, int ck,
-                                                         int mx, int my, int mz)
+                                                         int mx, int my)
 {
-    const long t = long(blockIdx.x) * KB + threadIdx.x;
-    const long rows = long(mx) * my * mz;
-    if (t >= rows) return;
-    const int ti = int(t % mx), tj = int((t / mx) % my), tk = int(t / (long(mx) * my));
-    const int i = ci + 3 * ti, j = cj + 3 * tj, k = ck + 3 * tk;
+    const int ti = int(blockIdx.x) * 64 + int(threadIdx.x & 63);
+    const int tj = __builtin_amdgcn_readfirstlane(int(blockIdx.y) * 4 + int(threadIdx.x >> 6));
+    const int tk = blockIdx.z;
+    if (tj >= my) return;
+    const bool act = ti < mx;  // lanes past the last row compute a copy and write nothing
+    const int i = ci + 3 * min(ti, mx - 1), j = cj + 3 * tj, k = ck + 3 * tk;
     const int nx = op.nx, ny = op.ny, nz = op.nz;
     const long sy = nx, sz = long(nx) * ny;
     const long ip = i + j * sy + k * sz;
-    cpx<double> lo[3], hi[3];
-    if constexpr (!ADJ) {
-        lo[0] = tab(op.xlo, i + 1); hi[0] = tab(op.xhi, i + 1);
-        lo[1] = tab(op.ylo, j + 1); hi[1] = tab(op.yhi, j + 1);
-        lo[2] = tab(op.zlo, k + 1); hi[2] = tab(op.zhi, k + 1);
+    // PML values at coordinate + d (d = -1, 0, 1) per axis; the forward row
+    // uses only its own (d = 0), the adjoint row its neighbours'
+    cpx<double> lo[3][3], hi[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!ADJ && d != 1) continue;
+        lo[0][d] = tab(op.xlo, i + d); hi[0][d] = tab(op.xhi, i + d);
+        lo[1][d] = tab(op.ylo, j + d); hi[1][d] = tab(op.yhi, j + d);
+        lo[2][d] = tab(op.zlo, k + d); hi[2][d] = tab(op.zhi, k + d);
     }
     const cpx<double> wp = wn_at(op, ip);
-    // coefficient of neighbour (dx,dy,dz) and its index (zero outside the grid)
-    auto coef_at = [&](int dx, int dy, int dz, long& q) -> cpx<double> {
+    cpx<double> cf[27], xv[27];
+    long qn[27];
+    bool in[27];
+#pragma unroll
+    for (int n = 0; n < 27; ++n) {
+        const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
         const int a = i + dx, bb = j + dy, c = k + dz;
-        const bool in = a >= 0 && a < nx && bb >= 0 && bb < ny && c >= 0 && c < nz;
-        q = in ? a + bb * sy + c * sz : ip;
-        if (!in) return cpx<double>{0.0, 0.0};
+        in[n] = a >= 0 && a < nx && bb >= 0 && bb < ny && c >= 0 && c < nz;
+        qn[n] = in[n] ? a + bb * sy + c * sz : ip;
         const int nnz = (dx != 0) + (dy != 0) + (dz != 0);
+        cpx<double> v;
         if constexpr (!ADJ) {
-            return cmul(cpx<double>{mass_mu(op, nnz), 0.0}, wn_at(op, q)) + stiff(op, dx, dy, dz, lo, hi);
+            const cpx<double> l3[3] = {lo[0][1], lo[1][1], lo[2][1]}, h3[3] = {hi[0][1], hi[1][1], hi[2][1]};
+            v = cmul(cpx<double>{mass_mu(op, nnz), 0.0}, wn_at(op, qn[n])) + stiff(op, dx, dy, dz, l3, h3);
         } else {
             // column p of H at row q, conjugated: xi of q, wn of p
-            cpx<double> ql[3], qh[3];
-            ql[0] = tab(op.xlo, a + 1); qh[0] = tab(op.xhi, a + 1);
-            ql[1] = tab(op.ylo, bb + 1); qh[1] = tab(op.yhi, bb + 1);
-            ql[2] = tab(op.zlo, c + 1); qh[2] = tab(op.zhi, c + 1);
-            return conjc(mass_mu(op, nnz) * wp + stiff(op, -dx, -dy, -dz, ql, qh));
+            const cpx<double> l3[3] = {lo[0][dx + 1], lo[1][dy + 1], lo[2][dz + 1]};
+            const cpx<double> h3[3] = {hi[0][dx + 1], hi[1][dy + 1], hi[2][dz + 1]};
+            v = conjc(mass_mu(op, nnz) * wp + stiff(op, -dx, -dy, -dz, l3, h3));
         }
-    };
+        cf[n] = in[n] ? v : cpx<double>{0.0, 0.0};
+        xv[n] = x[qn[n]];
+    }
     // residual of the row and |a|^2 (neighbours in the order dz, dy, dx)
     double atr = 0.0, ati = 0.0, nrm = 0.0;
-#pragma unroll 1
-    for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll 1
-        for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                long q;
-                const cpx<double> cf = coef_at(dx, dy, dz, q);
-                const cpx<double> xv = x[q];
-                atr += cf.re * xv.re - cf.im * xv.im;
-                ati += cf.re * xv.im + cf.im * xv.re;
-                nrm += cf.re * cf.re + cf.im * cf.im;
-            }
+    for (int n = 0; n < 27; ++n) {
+        atr += cf[n].re * xv[n].re - cf[n].im * xv[n].im;
+        ati += cf[n].re * xv[n].im + cf[n].im * xv[n].re;
+        nrm += cf[n].re * cf[n].re + cf[n].im * cf[n].im;
+    }
     const cpx<double> bp = b ? b[ip] : cpx<double>{0.0, 0.0};
     const double dr = omega * (bp.re - atr) / nrm, di = omega * (bp.im - ati) / nrm;
-    // x_q += d conj(a_q), coefficients formed again (registers over ALU)
-#pragma unroll 1
-    for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll 1
-        for (int dy = -1; dy <= 1; ++dy)
+    // x_q += d conj(a_q)
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                long q;
-                const cpx<double> cf = coef_at(dx, dy, dz, q);
-                if (cf.re != 0.0 || cf.im != 0.0) {
-                    cpx<double> v = x[q];
-                    v.re += dr * cf.re + di * cf.im;
-                    v.im += di * cf.re - dr * cf.im;
-                    x[q] = v;
-                }
-            }
+    for (int n = 0; n < 27; ++n) {
+        if (act && in[n] && (cf[n].re != 0.0 || cf[n].im != 0.0)) {
+            cpx<double> v = xv[n];
+            v.re += dr * cf[n].re + di * cf[n].im;
+            v.im += di * cf[n].re - dr * cf[n].im;
+            x[qn[n]] = v;
+        }
+    }
 }
 
 }  // namespace
@@ -157,15 +160,12 @@ hipError_t launch_kacz_sweep(bool adj, const OpDesc<double>& op, double omega, c
             const int c = pass == 0 ? cc : 26 - cc;
             const int ci = c % 3, cj = (c / 3) % 3, ck = c / 9;
             const int mx = (op.nx - ci + 2) / 3, my = (op.ny - cj + 2) / 3, mz = (op.nz - ck + 2) / 3;
-            const long rows = long(mx) * my * mz;
-            if (rows <= 0) continue;
-            const dim3 grid(unsigned((rows + KB - 1) / KB));
+            if (mx <= 0 || my <= 0 || mz <= 0) continue;
+            const dim3 grid(unsigned((mx + 63) / 64), unsigned((my + 3) / 4), unsigned(mz));
             if (adj)
-                hipLaunchKernelGGL(kacz_color_kernel<true>, grid, dim3(KB), 0, s, op, omega, x, b, ci, cj, ck, mx, my,
-                                   mz);
+                hipLaunchKernelGGL(kacz_color_kernel<true>, grid, dim3(KB), 0, s, op, omega, x, b, ci, cj, ck, mx, my);
             else
-                hipLaunchKernelGGL(kacz_color_kernel<false>, grid, dim3(KB), 0, s, op, omega, x, b, ci, cj, ck, mx,
-                                   my, mz);
+                hipLaunchKernelGGL(kacz_color_kernel<false>, grid, dim3(KB), 0, s, op, omega, x, b, ci, cj, ck, mx, my);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
