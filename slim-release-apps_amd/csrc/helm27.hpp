@@ -67,11 +67,12 @@ hipError_t launch_stencil(int variant, StencilLaunch<R> L, const StencilGrid& g,
 
 // One launch over nitems device-resident StencilLaunch items (operators
 // without DERIV); item it owns blocks [first[it], first[it+1]) of `blocks`,
-// launched with its own tiles_x/tiles_y/kchunk and the common shape (ty, pf).
+// launched with its own tiles_x/tiles_y/kchunk and the common shape (ty, pf,
+// ry: two rows per thread with ty = 8).
 // epi: 0 (y = alpha H x), 1 (+ beta b), 3, 4 (fused BiCGStab forms).
 // active (nullable): items with active[it] == 0 do nothing.
 template <typename R>
 hipError_t launch_stencil_batch(int variant, int epi, const StencilLaunch<R>* items, const int* first, int nitems,
-                                int blocks, const int* active, int ty, int pf, hipStream_t s);
+                                int blocks, const int* active, int ty, int pf, hipStream_t s, int ry = 1);
 
 }  // namespace helm3d

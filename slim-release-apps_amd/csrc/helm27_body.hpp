@@ -83,19 +83,16 @@ __device__ inline Cd cd_div(Cd a, Cd b)
 //      EPI 4 launch
 //   4: x = s = r - alpha v formed while staging, y = t = H s, partials
 //      (t, s), ||t||^2, (rhat, s) and (rhat, t)
-// (EPI 3 and 4 run one row per thread.)
 // The body runs one block `lin` of a launch of `G` blocks described by L.
 template <typename R, bool ADJ, bool DERIV, bool WNC, int TY, int PF, int EPI, int RY = 1>
 __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int lin, const int G)
 {
     static_assert(RY == 1 || RY == 2, "one or two rows per thread");
     static_assert(TY % RY == 0, "rows per thread divide the tile rows");
-    static_assert(RY == 1 || EPI <= 2, "the fused BiCGStab forms run one row per thread");
     constexpr int NT = TX * TY / RY;     // threads
     constexpr int HY = TY + 2;
     constexpr int NS = HX * HY;          // halo tile slots
     constexpr int NSL = (NS + NT - 1) / NT;  // load slots per thread
-    static_assert(EPI < 3 || NSL == 2, "fused BiCGStab staging assumes two slots per thread");
     static_assert(PF >= 1 && PF <= 4, "prefetch depth 1 to 4");
     if (L.skip && *L.skip) return;  // uniform over the launch
     const OpDesc<R>& op = L.op;
@@ -265,8 +262,11 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             ca = cpx<R>{R(-a.re), R(-a.im)};
         }
     }
-    cpx<R> Xlast = zero<R>();  // EPI 3/4: staged x at the own point of the previous plane
-    cpx<R> Rlast = zero<R>(), Dlast = zero<R>();  // EPI 3: r' and dz there
+    // EPI 3/4: staged x at the own points of the previous plane; EPI 3: r'
+    // and dz there
+    cpx<R> Xlast[RY], Rlast[RY], Dlast[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) Xlast[r] = Rlast[r] = Dlast[r] = zero<R>();
     // output scale: alpha, or alpha / sqrt(*anrm) with the norm held on the device
     const R alpha = (EPI <= 1 && L.anrm) ? R(double(L.alpha) / sqrt(*L.anrm)) : L.alpha;
 
@@ -280,7 +280,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         cpx<R> e[NSL];   // EPI 3: p, EPI 4: v at the slots
         cpx<R> f[NSL];   // EPI 3: v at the slots
         cpx<R> g[NSL];   // EPI 3: t at the slots
-        cpx<R> zo;       // EPI 3: z at own point of plane q-1
+        cpx<R> zo[RY];   // EPI 3: z at own points of plane q-1
         R w[NSL];        // forward, real wn at the slots
         cpx<R> c[NSL];   // forward, complex wn at the slots
         cpx<R> wc[RY];   // adjoint: wn at own points of plane q-1
@@ -317,7 +317,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             }
             if constexpr (EPI == 1) S.bb[r] = L.b[own];
             if constexpr (EPI == 3 || EPI == 4) S.bb[r] = L.rh[own];
-            if constexpr (EPI == 3) S.zo = L.zz[own];
+            if constexpr (EPI == 3) S.zo[r] = L.zz[own];
             if constexpr (EPI == 2) {
                 S.gu[r] = L.gu[own];
                 const int qk = min(max(q - 1 - L.p0[2], 0), L.pn[2] - 1);
@@ -385,7 +385,9 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             gu[r] = S.gu[r];
             gv[r] = S.gv[r];
         }
-        const cpx<R> zo = S.zo;
+        cpx<R> zo[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) zo[r] = S.zo[r];
         fetch(q + PF, S);  // unconditional: a fixed number of loads per step
         __syncthreads();
 
@@ -517,36 +519,36 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                 Pend[r] = Sdn;
                 MOcur[r] = Mo;
             }
-            // Exactly one store per step, lane and row (four for EPI 3, with
-            // one row): steps or lanes without an output write to the sink
-            // instead (keeps the vmcnt bookkeeping exact).
+            // Exactly one store per step, lane and row (four for EPI 3):
+            // steps or lanes without an output write to the sink instead
+            // (keeps the vmcnt bookkeeping exact).
             const bool live = emit && active[r];
             const int sk = r == 0 ? sink_slot : sink_slot2;
             if constexpr (EPI == 3) {
-                const long od = long(kc) * plane + offc[0];
+                const long od = long(kc) * plane + offc[r];
                 cpx<R>* dy = live ? &L.y[od] : &L.sink[sink_slot];
                 cpx<R>* d2 = live ? &L.o2[od] : &L.sink[sink_slot2];
                 cpx<R>* d3 = live ? &L.o3[od] : &L.sink[sink_slot3];
                 cpx<R>* d4 = live ? &L.zz[od] : &L.sink[sink_slot4];
                 *dy = out;
-                *d2 = Xlast;
-                *d3 = Rlast;
-                *d4 = zo + Dlast;
+                *d2 = Xlast[r];
+                *d3 = Rlast[r];
+                *d4 = zo[r] + Dlast[r];
                 if (live) {
                     // (rhat, v) += conj(rhat) v
-                    acc[0] += double(bb[0].re) * double(out.re) + double(bb[0].im) * double(out.im);
-                    acc[1] += double(bb[0].re) * double(out.im) - double(bb[0].im) * double(out.re);
+                    acc[0] += double(bb[r].re) * double(out.re) + double(bb[r].im) * double(out.im);
+                    acc[1] += double(bb[r].re) * double(out.im) - double(bb[r].im) * double(out.re);
                 }
-                Xlast = su[buf][tw + 1][tx + 1];  // own point of plane q
-                Rlast = sr[buf][tw + 1][tx + 1];
-                Dlast = sd[buf][tw + 1][tx + 1];
+                Xlast[r] = su[buf][tr0 + r + 1][tx + 1];  // own point of plane q
+                Rlast[r] = sr[buf][tr0 + r + 1][tx + 1];
+                Dlast[r] = sd[buf][tr0 + r + 1][tx + 1];
             } else if constexpr (EPI == 4) {
-                cpx<R>* dy = live ? &L.y[long(kc) * plane + offc[0]] : &L.sink[sink_slot];
+                cpx<R>* dy = live ? &L.y[long(kc) * plane + offc[r]] : &L.sink[sk];
                 *dy = out;
                 if (live) {
                     // (t, s) += conj(t) s ; ||t||^2 ; (rhat, s) ; (rhat, t)
-                    const double tr_ = out.re, ti = out.im, sr_ = Xlast.re, si = Xlast.im, hr = bb[0].re,
-                                 hi = bb[0].im;
+                    const double tr_ = out.re, ti = out.im, sr_ = Xlast[r].re, si = Xlast[r].im, hr = bb[r].re,
+                                 hi = bb[r].im;
                     acc[0] += tr_ * sr_ + ti * si;
                     acc[1] += tr_ * si - ti * sr_;
                     acc[2] += tr_ * tr_ + ti * ti;
@@ -555,7 +557,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                     acc[5] += hr * tr_ + hi * ti;
                     acc[6] += hr * ti - hi * tr_;
                 }
-                Xlast = su[buf][tw + 1][tx + 1];  // own point of plane q
+                Xlast[r] = su[buf][tr0 + r + 1][tx + 1];  // own point of plane q
             } else if constexpr (EPI == 2) {
                 const int qk = kc - L.p0[2];
                 const bool g = live && gcol[r] && qk >= 0 && qk < L.pn[2];

@@ -119,7 +119,7 @@ struct Layout {
     int n = 0, m = 0;
     bool mixed = false;
     int variant = 0;
-    int ty = 4, pf = 2;
+    int ty = 4, pf = 2, ry = 1;
     std::vector<long> N, ld, off64, off32;
     std::vector<int> nblk, sblk, first;
     std::vector<StencilGrid> grid;
@@ -230,6 +230,7 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
     Lo.variant = (mode != 1 ? VAR_ADJ : VAR_FORW) | (wnc ? VAR_WNC : 0);
     Lo.ty = ctxs[0]->grid.ty;
     Lo.pf = ctxs[0]->grid.pf;
+    Lo.ry = ctxs[0]->grid.ry;
     const int slots64 = (m + 1) + (Lo.mixed ? m : 0);
     const int slots32 = Lo.mixed ? S32_COUNT : 0;
     long tot64 = 0, tot32 = 0;
@@ -403,10 +404,10 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
             const StencilLaunch<float>* d1 =
                 it == 0 ? d_k1first : at<StencilLaunch<float>>(B, Lo.o_k1) + size_t(par) * n;
             HIPCHK(launch_stencil_batch<float>(Lo.variant, 3, d1, d_first, n, Lo.sblk_total, d_active, Lo.ty,
-                                               Lo.pf, s));
+                                               Lo.pf, s, Lo.ry));
             HIPCHK(launch_breduce<float>(d_i32, d_active, n, 2, BS_RV, BT_SC, true, s));
             HIPCHK(launch_stencil_batch<float>(Lo.variant, 4, at<StencilLaunch<float>>(B, Lo.o_k2) + size_t(par) * n,
-                                               d_first, n, Lo.sblk_total, d_active, Lo.ty, Lo.pf, s));
+                                               d_first, n, Lo.sblk_total, d_active, Lo.ty, Lo.pf, s, Lo.ry));
             HIPCHK(launch_breduce<float>(d_i32, d_active, n, 7, BS_TS, BT_SC, true, s));
         }
         const int last = int(inner_its & 1);  // slots of the last iteration's p, r, v
@@ -512,7 +513,7 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
                 HIPCHK(launch_bup(d_i32, d_i64, d_active, n, nbm, S32_Z, m + 1 + (k - 1), st));
             }
             HIPCHK(launch_stencil_batch<double>(Lo.variant, 0, at<StencilLaunch<double>>(B, Lo.o_st64) + size_t(k) * n,
-                                                d_first, n, Lo.sblk_total, d_active, Lo.ty, Lo.pf, st));
+                                                d_first, n, Lo.sblk_total, d_active, Lo.ty, Lo.pf, st, Lo.ry));
             // classical Gram-Schmidt (FGMRES.m:100-103) on the unnormalised basis
             // streaming basis loads once the slots read per pass exceed the last-level cache
             const bool nt = double(k) * double(Lo.total_points) * sizeof(cpx<double>) > 256.0 * (1 << 20);
@@ -570,7 +571,7 @@ int helm3d_batch_solve(helm3d_batch* B, int n, helm3d_ctx* const* ctxs, int mode
         // true residual r = b - A x into slot 0 and its norm (FGMRES.m:120-124)
         CHK(set_active([&](int i) { return !S[i].done; }));
         HIPCHK(launch_stencil_batch<double>(Lo.variant, 1, at<StencilLaunch<double>>(B, Lo.o_res64), d_first, n,
-                                            Lo.sblk_total, d_active, Lo.ty, Lo.pf, st));
+                                            Lo.sblk_total, d_active, Lo.ty, Lo.pf, st, Lo.ry));
         HIPCHK(launch_breduce<double>(d_i64, d_active, n, 1, kNrm, BT_SMALL, true, st));
         CHK(fetch());
         bool all_below = true;

@@ -44,37 +44,41 @@ class BatchSolver:
             pass
 
     def autotune(self, ops: Sequence[Helm3d], bs: Sequence[torch.Tensor], mode: int = 1, method: str = "mixed",
-                 inner_maxit: int = 25, tile_rows=(4, 8), prefetch=(1, 2), divisors=(1, 2, 3, 4)) -> dict:
+                 inner_maxit: int = 25, tile_rows=(4, 8), prefetch=(1, 2), divisors=(1, 2, 3, 4),
+                 rows_per_thread=(1, 2)) -> dict:
         """Choose the stencil launch shape of a batch by timing a short batched
         solve (one restart cycle of two outer steps) for every candidate: 64 x
-        tile_rows column tiles, prefetch depth, and z chunks of nz/divisor for
-        every operator.  The fastest shape is left set on the operators; the
-        results of later solves do not depend on it."""
+        tile_rows column tiles (64 x 8 with one or two rows per thread),
+        prefetch depth, and z chunks of nz/divisor for every operator.  The
+        fastest shape is left set on the operators; later solves converge to
+        the same tolerance whatever the shape (the fused BiCGStab partial sums
+        follow the shape, so iteration counts can move by a few)."""
         uniq = list({id(H): H for H in ops}.values())
         best = None
         for ty in tile_rows:
-            for pf in prefetch:
-                for d in divisors:
-                    for H in uniq:
-                        H.set_launch((H.n[2] + d - 1) // d, ty, pf)
-                    torch.cuda.synchronize(self.device)
-                    t = []
-                    for _ in range(2):
-                        e0 = torch.cuda.Event(enable_timing=True)
-                        e1 = torch.cuda.Event(enable_timing=True)
-                        e0.record()
-                        self.solve(ops, bs, mode=mode, method=method, tol=1e-30, maxinnerit=2, maxit=1,
-                                   inner_maxit=inner_maxit)
-                        e1.record()
+            for ry in (rows_per_thread if ty == 8 else (1,)):
+                for pf in prefetch:
+                    for d in divisors:
+                        for H in uniq:
+                            H.set_launch((H.n[2] + d - 1) // d, ty, pf, ry)
                         torch.cuda.synchronize(self.device)
-                        t.append(e0.elapsed_time(e1))
-                    cand = (min(t), ty, pf, d)
-                    if best is None or cand < best:
-                        best = cand
-        _, ty, pf, d = best
+                        t = []
+                        for _ in range(2):
+                            e0 = torch.cuda.Event(enable_timing=True)
+                            e1 = torch.cuda.Event(enable_timing=True)
+                            e0.record()
+                            self.solve(ops, bs, mode=mode, method=method, tol=1e-30, maxinnerit=2, maxit=1,
+                                       inner_maxit=inner_maxit)
+                            e1.record()
+                            torch.cuda.synchronize(self.device)
+                            t.append(e0.elapsed_time(e1))
+                        cand = (min(t), ty, ry, pf, d)
+                        if best is None or cand < best:
+                            best = cand
+        _, ty, ry, pf, d = best
         for H in uniq:
-            H.set_launch((H.n[2] + d - 1) // d, ty, pf)
-        return dict(tile_rows=ty, prefetch=pf, divisor=d, ms=best[0])
+            H.set_launch((H.n[2] + d - 1) // d, ty, pf, ry)
+        return dict(tile_rows=ty, rows_per_thread=ry, prefetch=pf, divisor=d, ms=best[0])
 
     @staticmethod
     def item_bytes(H: Helm3d, method: str = "mixed", maxinnerit: int = 20) -> int:

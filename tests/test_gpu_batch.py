@@ -95,6 +95,35 @@ def test_mixed_batch_uneven_grids(mode):
         assert err < 1e-3
 
 
+@pytest.mark.parametrize("method", ["mixed", "plain"])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_two_rows_per_thread(mode, method):
+    """Batched solves on the 64 x 8, two-rows-per-thread launch shape (the
+    fused BiCGStab passes and the FP64 work-list stencils with two output rows
+    per thread): every item reaches tol with the oracle operator and agrees
+    with a complex128 solve, on grids whose y extent is not a multiple of 8."""
+    import torch
+    from helm3d import BatchSolver
+    c1, w1, H1 = _op((32, 30, 24), 8, 5.0, 14)
+    c2, w2, H2 = _op((48, 37, 28), 8, 7.0, 15)
+    H1.set_launch(0, 8, 2, 2)
+    H2.set_launch(13, 8, 2, 2)
+    items = [(c1, w1, H1, (16, 16, 3)), (c2, w2, H2, (24, 18, 3)), (c2, w2, H2, (8, 30, 20))]
+    bs = [_rhs(c, s) for c, _, _, s in items]
+    print("batch two rows per thread, mode", mode, method, flush=True)
+    S = BatchSolver("cuda:0")
+    xs, infos = S.solve([it[2] for it in items], [torch.from_numpy(b).cuda() for b in bs], mode=mode,
+                        method=method, tol=1e-6, inner_maxit=10)
+    for (c, w, H, _), b, x, inf in zip(items, bs, xs, infos):
+        xn = x.cpu().numpy()
+        rel = _relres(c, w, b, xn, mode)
+        xr, _ = H.solve(torch.from_numpy(b).cuda(), mode=mode, tol=1e-10)
+        err = np.linalg.norm(xn - xr.cpu().numpy()) / np.linalg.norm(xr.cpu().numpy())
+        print("item iters", inf.iters, "relres", inf.relres, rel, "err", err, flush=True)
+        assert inf.converged and rel < 1e-6
+        assert err < 1e-3
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_mixed_batch_complex_wavenumber(mode):
     """Fused complex64 BiCGStab on the complex-wn operators (WNC variants):
@@ -166,7 +195,8 @@ def test_batch_autotune_keeps_results():
     r = S.autotune([H, H], [b, 2 * b], inner_maxit=8, divisors=(1, 2))
     print("batch autotune", r, flush=True)
     L = H.get_launch()
-    assert (L["tile_rows"], L["prefetch"]) == (r["tile_rows"], r["prefetch"])
+    assert (L["tile_rows"], L["prefetch"], L["rows_per_thread"]) == (r["tile_rows"], r["prefetch"],
+                                                                     r["rows_per_thread"])
     x1, i1 = S.solve([H, H], [b, 2 * b], method="mixed", tol=1e-6, inner_maxit=8)
     # the dot products reduced in the stencil epilogues follow the block
     # decomposition, so another shape changes rounding only
