@@ -157,8 +157,7 @@ int helm3d_apply_noalloc(int variant, const int n[3], const double h[3], const i
  * most 2) and rows per thread (1, or 2 with 8 tile rows: 256 threads per
  * 64 x 8 tile instead of 512, so two tiles share a compute unit).  The
  * result is identical for every shape; only speed changes.  Chosen per grid
- * by the autotuner (helm3d_autotune); the batched kernels run one row per
- * thread. */
+ * by the autotuner (helm3d_autotune). */
 int helm3d_set_launch(helm3d_ctx *ctx, int kchunk, int tile_rows, int prefetch, int rows_per_thread);
 int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetch, int *rows_per_thread,
                       int *blocks);
@@ -169,7 +168,7 @@ int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetc
  * `reps` launches each with HIP events on the context's stream, and keeps the
  * fastest (results are identical for every shape).  No counterpart in the
  * reference; the batched solves use each item's tuned z chunk and the first
- * item's tile rows and prefetch. */
+ * item's tile rows, rows per thread and prefetch. */
 int helm3d_autotune(helm3d_ctx *ctx, int mode, int precision, int reps, int *kchunk, int *tile_rows,
                     int *prefetch, int *rows_per_thread, double *best_ms);
 
