@@ -129,11 +129,11 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     const int tr0 = tw * RY;                       // first tile row of this thread
     const int i = i0 + tx;
     const int ic = min(i, nx - 1);  // clamped for table reads
-    // A wave is RY rows of 64 columns.  Per row it takes one of four paths
+    // A wave is RY rows of 64 columns.  Per row it takes one of two paths
     // (uniform over the wave, and the same for an output point whatever the
     // launch shape): interior when every column and the row have
-    // xi_lo = xi_hi = 1 at index-1, index, index+1; x PML only or y PML only
-    // when one axis has; the general path otherwise.
+    // xi_lo = xi_hi = 1 at index-1, index, index+1, the general path
+    // otherwise.
     const int jw = __builtin_amdgcn_readfirstlane(j0 + tr0);
     const bool xint = i0 >= op.int_lo[0] && i0 + TX - 1 <= op.int_hi[0];
     bool yint[RY], active[RY];
