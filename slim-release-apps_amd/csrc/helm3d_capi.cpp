@@ -187,6 +187,14 @@ int cycle_graph(helm3d_ctx* c, int variant, int m, const cpx<R>* V, long ld, int
 // step.  The host builds the Hessenberg columns and quasi-residuals in order;
 // steps enqueued past the stopping point are discarded (their slots are never
 // combined), so the arithmetic is that of the one-step-at-a-time loop.
+//
+// Calls without P that want only x (multigrid smoothers and coarsest-level
+// solves, at most kMaxDeviceCycles cycles) never meet the host: every cycle
+// the loop bound allows is enqueued, and device kernels take the decisions --
+// the least squares per cycle (Givens rotations as the columns come, the
+// quasi-residual |g_k| / ||r0|| of the same least squares), the true residual
+// per restart, a zero initial residual -- with the kernels of the cycles
+// after the stop reading a device flag and returning.
 template <typename R>
 int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool use_x0, double tol, int m,
                  long maxit, long iter_cap, PrecondT<R>* P, GmresResult* res, std::vector<double>* hst,
