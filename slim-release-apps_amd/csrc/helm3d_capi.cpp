@@ -230,9 +230,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         CHK(stencil<R>(c, variant, op, x, slot(0), R(-1), b, R(1), c->part));
         TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, c->grid.blocks, 1, c->nrm, c->stream, c->small_host_dev));
     } else {
-        HIPCHK((launch_axpby<R, R>(1.0, 0.0, b, 0.0, 0.0, slot(0), N, false, c->stream)));
-        HIPCHK((launch_axpby<R, R>(0.0, 0.0, static_cast<const cpx<R>*>(nullptr), 0.0, 0.0, x, N, false, c->stream)));
-        HIPCHK(launch_norm2<R>(slot(0), N, c->part, nblk, c->stream));
+        HIPCHK(launch_start<R>(b, slot(0), x, N, c->part, nblk, c->stream));
         TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm, c->stream, c->small_host_dev));
     }
     // enqueue Arnoldi step j: slot(j) = A slot(j-1)/||slot(j-1)||, dots into

@@ -163,29 +163,36 @@ __global__ void gmres_restart_kernel(const double* __restrict__ nrm, double tol,
 // (stride ldr) hold the dots d_0..d_{j-1} of step j and ||w_j||^2; the
 // Hessenberg column of step j is sig_q d_q (q < j) over ||w_j||, with the
 // basis slot scales sig_q = 1/||v_q||.  The columns are reduced as they come
-// by Givens rotations (the triangular factor and the rotated right-hand side
-// in LDS), so the quasi-residual of step k is |g_k| / ||r0|| -- the residual
+// by Givens rotations (the triangular factor T and the rotated right-hand
+// side g), so the quasi-residual of step k is |g_k| / ||r0|| -- the residual
 // of the k-step least squares -- after O(k) work.  Steps after the first one
 // at or below tol are dropped (the host loop would not have run them); coef
 // receives a_q = y_q sig_q for the kept steps, zeros after.  One thread.
-__global__ void gmres_lsq_kernel(const double* __restrict__ rows, int ldr, int m, double tol, GmresDevState* st)
+// Cycles of at most MB steps; MB = kMaxBasis keeps T, g and the rotations in
+// LDS, smaller MB in registers (every index fixed once the loops unroll).
+template <int MB>
+__device__ __forceinline__ void gmres_lsq_body(const double* __restrict__ rows, int ldr, int m, double tol,
+                                               GmresDevState* st, Cx* T, Cx* cs, Cx* sn, Cx* g, Cx* y,
+                                               double* sig)
 {
-    constexpr int ld = kMaxBasis + 1;
-    __shared__ Cx T[ld * kMaxBasis];  // triangular factor, column-major
-    __shared__ Cx cs[kMaxBasis], sn[kMaxBasis], g[kMaxBasis + 1], y[kMaxBasis];
-    __shared__ double sig[kMaxBasis + 1];
-    if (threadIdx.x != 0 || st->skip) return;
+    constexpr int ld = MB + 1;
+    // unrolled (register arrays) for short cycles, rolled for the LDS form
+    constexpr int kU = MB <= 5 ? MB + 1 : 1;
     const double normr0 = st->normr0;
     sig[0] = 1.0 / st->beta;
     g[0] = Cx{st->beta, 0.0};
     int kept = m;
     double hlast = st->hlast;
-    for (int k = 1; k <= m; ++k) {
+#pragma unroll kU
+    for (int k = 1; k <= MB; ++k) {
+        if (k > m) break;
         const double* h = rows + long(k) * ldr;
         Cx* col = T + (k - 1) * ld;
+#pragma unroll kU
         for (int q = 0; q < k; ++q) col[q] = cx_scale(Cx{h[2 * q], h[2 * q + 1]}, sig[q]);
         const double hn = sqrt(h[2 * k]);
         sig[k] = 1.0 / hn;
+#pragma unroll kU
         for (int q = 0; q + 1 < k; ++q) {
             const Cx u = col[q], v = col[q + 1];
             col[q] = cx_add(cx_mul(cx_conj(cs[q]), u), cx_mul(cx_conj(sn[q]), v));
@@ -211,19 +218,42 @@ __global__ void gmres_lsq_kernel(const double* __restrict__ rows, int ldr, int m
             break;
         }
     }
-    for (int r = kept - 1; r >= 0; --r) {
-        Cx acc = g[r];
-        for (int c = r + 1; c < kept; ++c) acc = cx_sub(acc, cx_mul(T[r + c * ld], y[c]));
-        y[r] = cx_div(acc, T[r + r * ld]);
+#pragma unroll kU
+    for (int r = MB - 1; r >= 0; --r) {
+        if (r < kept) {
+            Cx acc = g[r];
+#pragma unroll kU
+            for (int c = r + 1; c < MB; ++c)
+                if (c < kept) acc = cx_sub(acc, cx_mul(T[r + c * ld], y[c]));
+            y[r] = cx_div(acc, T[r + r * ld]);
+        }
     }
+#pragma unroll kU
     for (int q = 0; q < kMaxBasis; ++q) {
-        const Cx a = q < kept ? cx_scale(y[q], sig[q]) : Cx{0.0, 0.0};
+        const Cx a = (q < MB && q < kept) ? cx_scale(y[q < MB ? q : 0], sig[q < MB ? q : 0]) : Cx{0.0, 0.0};
         st->coef[2 * q] = a.re;
         st->coef[2 * q + 1] = a.im;
     }
     st->hlast = hlast;
     st->kept = kept;
     if (hlast <= tol) st->stop = 1;
+}
+
+template <int MB>
+__global__ __launch_bounds__(64) void gmres_lsq_kernel(const double* __restrict__ rows, int ldr, int m, double tol,
+                                                       GmresDevState* st)
+{
+    if (threadIdx.x != 0 || st->skip) return;
+    if constexpr (MB == kMaxBasis) {
+        __shared__ Cx T[(MB + 1) * MB];
+        __shared__ Cx cs[MB], sn[MB], g[MB + 1], y[MB];
+        __shared__ double sig[MB + 1];
+        gmres_lsq_body<MB>(rows, ldr, m, tol, st, T, cs, sn, g, y, sig);
+    } else {
+        Cx T[(MB + 1) * MB], cs[MB], sn[MB], g[MB + 1], y[MB];
+        double sig[MB + 1];
+        gmres_lsq_body<MB>(rows, ldr, m, tol, st, T, cs, sn, g, y, sig);
+    }
 }
 
 // out[q] = sum_b part[q*nblk + b] in block order; one block per column q
@@ -265,6 +295,26 @@ __global__ __launch_bounds__(KB) void norm2_kernel(const cpx<R>* __restrict__ x,
                                                    double* __restrict__ part)
 {
     kb::norm2_body<R>(x, n, part, blockIdx.x, gridDim.x);
+}
+
+// Start of a solve from x = 0 in one pass: w = b, x = 0 and the partial
+// ||b||^2 of norm2_kernel (same blocks, same order).
+template <typename R>
+__global__ __launch_bounds__(KB) void start_kernel(const cpx<R>* __restrict__ b, cpx<R>* __restrict__ w,
+                                                   cpx<R>* __restrict__ x, long n, double* __restrict__ part)
+{
+    __shared__ double red[KB / 64];
+    double s2[1] = {0.0};
+    long lo, hi;
+    kb::range_even(n, blockIdx.x, gridDim.x, lo, hi);
+    for (long t = lo + threadIdx.x; t < hi; t += KB) {
+        const cpx<R> v = b[t];
+        w[t] = v;
+        x[t] = cpx<R>{R(0), R(0)};
+        s2[0] += double(v.re) * double(v.re) + double(v.im) * double(v.im);
+    }
+    kb::block_sum<1>(s2, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = s2[0];
 }
 
 // Split real/imaginary arrays (the MEX boundary layout) <-> interleaved complex.
@@ -389,7 +439,12 @@ hipError_t launch_gmres_restart(const double* nrm, double tol, GmresDevState* st
 hipError_t launch_gmres_lsq(const double* rows, int ldr, int m, double tol, GmresDevState* st, hipStream_t s)
 {
     if (m < 1 || m >= kMaxBasis) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(gmres_lsq_kernel, dim3(1), dim3(1), 0, s, rows, ldr, m, tol, st);
+    if (m <= 2)
+        hipLaunchKernelGGL(gmres_lsq_kernel<2>, dim3(1), dim3(1), 0, s, rows, ldr, m, tol, st);
+    else if (m <= 5)
+        hipLaunchKernelGGL(gmres_lsq_kernel<5>, dim3(1), dim3(1), 0, s, rows, ldr, m, tol, st);
+    else
+        hipLaunchKernelGGL(gmres_lsq_kernel<kMaxBasis>, dim3(1), dim3(1), 0, s, rows, ldr, m, tol, st);
     return hipGetLastError();
 }
 
@@ -446,7 +501,15 @@ hipError_t launch_norm2(const cpx<R>* x, long n, double* part, int nblk, hipStre
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_start(const cpx<R>* b, cpx<R>* w, cpx<R>* x, long n, double* part, int nblk, hipStream_t s)
+{
+    hipLaunchKernelGGL((start_kernel<R>), dim3(nblk), dim3(KB), 0, s, b, w, x, n, part);
+    return hipGetLastError();
+}
+
 #define HELM3D_KRYLOV_INST(R)                                                                               \
+    template hipError_t launch_start<R>(const cpx<R>*, cpx<R>*, cpx<R>*, long, double*, int, hipStream_t);  \
     template hipError_t launch_dots<R>(const cpx<R>*, long, int, const cpx<R>*, long, double*, int,        \
                                        hipStream_t, const int*);                                            \
     template hipError_t launch_update<R>(const cpx<R>*, long, int, const double*, const CoefPack&, cpx<R>*,  \

@@ -68,5 +68,8 @@ hipError_t launch_axpby(double ar, double ai, const cpx<RX>* x, double br, doubl
                         bool use_y, hipStream_t s);
 template <typename R>
 hipError_t launch_norm2(const cpx<R>* x, long n, double* part, int nblk, hipStream_t s);
+// w = b, x = 0 and the partials of ||b||^2 (those of launch_norm2 on w) in one pass.
+template <typename R>
+hipError_t launch_start(const cpx<R>* b, cpx<R>* w, cpx<R>* x, long n, double* part, int nblk, hipStream_t s);
 
 }  // namespace helm3d
