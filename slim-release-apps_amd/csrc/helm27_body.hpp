@@ -19,10 +19,11 @@
 // The adjoint is the exact conjugate transpose: F/G take the neighbour's x/y
 // PML values and the z weights come from the neighbouring planes.
 //
-// Tiling.  A 256-thread block owns a 64 x 4 column tile and marches over a z
-// chunk.  Each plane (with a one-point x/y halo) is staged once in LDS
-// (double-buffered, one barrier per plane); the next plane's global loads are
-// issued before the barrier so HBM latency overlaps the current plane.  Block
+// Tiling.  A block of 64 x TY / RY threads owns a 64 x TY column tile and
+// marches over a z chunk.  Each plane (with a one-point x/y halo) is staged
+// once in LDS (double-buffered, one barrier per plane); the loads of the
+// planes ahead are issued before the barrier so HBM latency overlaps the
+// current plane.  Block
 // ids are remapped so consecutive tiles of the same chunk land on the same XCD
 // and share halo lines in its L2.
 #pragma once
@@ -152,13 +153,14 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
     // factored form: six complex multipliers times real-weighted sums of the
     // plane's 3x3 neighbourhood (24 registers instead of 18 complex
     // coefficients).  Adjoint: side values of the neighbours, conjugated.
-    // The y multipliers are uniform over the wave (rows jw .. jw+RY-1): read
-    // at wave-uniform indices so they live in scalar registers.
+    // Forward, the face multipliers are the sums of the side ones and are not
+    // formed.  The y multipliers are uniform over the wave (rows jw ..
+    // jw+RY-1): read at wave-uniform indices so they live in scalar registers.
     cpx<R> px0{}, px2{}, pxf{};
     cpx<R> py0[RY], py2[RY], pyf[RY];
     if constexpr (!DERIV) {
         if constexpr (!ADJ) {
-            px0 = op.xlo[ic + 1]; px2 = op.xhi[ic + 1]; pxf = px0 + px2;
+            px0 = op.xlo[ic + 1]; px2 = op.xhi[ic + 1];  // face multiplier: px0 + px2
         } else {
             // neighbour i-1 couples through its hi value, i+1 through its lo value
             px0 = conjc(op.xhi[ic]); px2 = conjc(op.xlo[ic + 2]); pxf = conjc(op.xlo[ic + 1] + op.xhi[ic + 1]);
@@ -167,7 +169,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
         for (int r = 0; r < RY; ++r) {
             const int jcw = min(jw + r, ny - 1);
             if constexpr (!ADJ) {
-                py0[r] = op.ylo[jcw + 1]; py2[r] = op.yhi[jcw + 1]; pyf[r] = py0[r] + py2[r];
+                py0[r] = op.ylo[jcw + 1]; py2[r] = op.yhi[jcw + 1];
             } else {
                 py0[r] = conjc(op.yhi[jcw]); py2[r] = conjc(op.ylo[jcw + 2]);
                 pyf[r] = conjc(op.ylo[jcw + 1] + op.yhi[jcw + 1]);
@@ -437,7 +439,14 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             Mo = rfma(Mo, -op.wm4, md);
 
             // ---- stiffness: Qz (z PML, applied per plane) and F, G (x/y PML)
+            // Forward: F and G accumulate onto the mass terms of the same
+            // planes (FM = F + Mc, GM = G + Mo); adjoint: the mass terms are
+            // weighted by conj(wn) at the output and stay apart.
             cpx<R> Qn = zero<R>(), Fn = zero<R>(), Gn = zero<R>();
+            if constexpr (!ADJ) {
+                Fn = Mc;
+                Gn = Mo;
+            }
             if constexpr (!DERIV) {
                 const cpx<R> u11 = u[1][r + 1];
                 // pair sums along y (column a) and along x (row b)
@@ -452,14 +461,38 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                 if (xint && yint[r]) {
                     // x/y PML absent around this wave: F, G are real constant
                     // stencils on the sums already formed for Qz
-                    Fn = op.fi[0] * u11;
+                    Fn = rfma(Fn, op.fi[0], u11);
                     Fn = rfma(Fn, op.fi[1], sx1);
                     Fn = rfma(Fn, op.fi[2], sy1);
                     Fn = rfma(Fn, op.fi[3], dg);
-                    Gn = op.gi[0] * u11;
+                    Gn = rfma(Gn, op.gi[0], u11);
                     Gn = rfma(Gn, op.gi[1], sx1);
                     Gn = rfma(Gn, op.gi[2], sy1);
                     Gn = rfma(Gn, op.gi[3], dg);
+                } else if constexpr (!ADJ) {
+                    // Forward: the face multipliers are the sums of the side
+                    // ones (pxf = px0 + px2, pyf = py0 + py2), so each face
+                    // term folds into the two side terms, which then act on
+                    // differences against the centre row / column shared by
+                    // F and G:
+                    //   F = px0 (cx d0u + w_xy d0s) + px2 (cx d2u + w_xy d2s)
+                    //     + py0 (cy e0u + w_xy e0s) + py2 (cy e2u + w_xy e2s)
+                    //   G = px0 (-w_xz d0u + w3a2 d0s) + ... + py2 (-w_yz e2u + w3a2 e2s)
+                    // (the terms listed in the adjoint branch below with
+                    // pxf = px0 + px2 and pyf = py0 + py2, regrouped).
+                    const cpx<R> sy0 = u[0][r] + u[0][r + 2], sy2 = u[2][r] + u[2][r + 2];
+                    const cpx<R> d0u = u[0][r + 1] - u11, d2u = u[2][r + 1] - u11;
+                    const cpx<R> d0s = sy1 - sy0, d2s = sy1 - sy2;
+                    const cpx<R> e0u = u[1][r] - u11, e2u = u[1][r + 2] - u11;
+                    const cpx<R> e0s = sx1 - sx0, e2s = sx1 - sx2;
+                    Fn = cfma(Fn, px0, rfma(cx * d0u, wxy, d0s));
+                    Gn = cfma(Gn, px0, rfma((-wxz) * d0u, w3, d0s));
+                    Fn = cfma(Fn, px2, rfma(cx * d2u, wxy, d2s));
+                    Gn = cfma(Gn, px2, rfma((-wxz) * d2u, w3, d2s));
+                    Fn = cfma(Fn, py0[r], rfma(cy * e0u, wxy, e0s));
+                    Gn = cfma(Gn, py0[r], rfma((-wyz) * e0u, w3, e0s));
+                    Fn = cfma(Fn, py2[r], rfma(cy * e2u, wxy, e2s));
+                    Gn = cfma(Gn, py2[r], rfma((-wyz) * e2u, w3, e2s));
                 } else {
                     // F: centre -(cx xf + cy yf); x faces cx xs + w_xy yf; y faces
                     //    cy ys + w_xy xf; diagonals -w_xy (xs + ys)
@@ -467,7 +500,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                     //    y faces -w_yz ys + w3a2 xf; diagonals -w3a2 (xs + ys)
                     // (the non-xi_z part of Helm3d_27pt.h:680-714), grouped by
                     // multiplier; sy0/sy2: column sums, sx0/sx2: row sums of the
-                    // diagonal pairs.
+                    // diagonal pairs.  Adjoint: the face multipliers are the
+                    // column's own values, not sums of the side ones.
                     const cpx<R> sy0 = u[0][r] + u[0][r + 2], sy2 = u[2][r] + u[2][r + 2];
                     // accumulated multiplier by multiplier (few live temporaries)
                     Fn = cmul(px0, rfma(cx * u[0][r + 1], -wxy, sy0));
@@ -488,7 +522,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             cpx<R> out = zero<R>();
             if constexpr (!ADJ) {
                 // y(kc) = wlo Q(kc-1) + whi Q(kc+1) - (wlo+whi) Q(kc) + GM(kc-1) + GM(kc+1) + FM(kc)
-                const cpx<R> GMn = Gn + Mo;
+                const cpx<R> GMn = Gn;  // G + Mo
                 if (emit) {
                     out = Acc[r] + GMn;
                     if constexpr (!DERIV) {
@@ -499,7 +533,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                 }
                 Qprev[r] = Qcur[r];
                 Qcur[r] = Qn;
-                Acc[r] = Pend[r] + (Fn + Mc);
+                Acc[r] = Pend[r] + Fn;  // GM(kc-1) + F + Mc
                 Pend[r] = GMn;
             } else {
                 // contributions of plane q to outputs q+1 (Sdn), q-1 (Sup), q (S0)
