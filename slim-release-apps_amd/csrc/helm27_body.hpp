@@ -431,7 +431,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
             // ---- mass-lumping sums (forward: on wn .* x, adjoint: on x)
             const cpx<R> mf = xsm[r + 1] + (m1[r] + m1[r + 2]);
             const cpx<R> md = xsm[r] + xsm[r + 2];
-            cpx<R> Mc = op.wn_coef * m1[r + 1];
+            cpx<R> Mc = ADJ ? rfma(MOcur[r], op.wn_coef, m1[r + 1]) : op.wn_coef * m1[r + 1];
             Mc = rfma(Mc, -op.wm2, mf);
             Mc = rfma(Mc, -op.wm3, md);
             cpx<R> Mo = (-op.wm2) * m1[r + 1];
@@ -440,12 +440,16 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
 
             // ---- stiffness: Qz (z PML, applied per plane) and F, G (x/y PML)
             // Forward: F and G accumulate onto the mass terms of the same
-            // planes (FM = F + Mc, GM = G + Mo); adjoint: the mass terms are
-            // weighted by conj(wn) at the output and stay apart.
+            // planes (FM = F + Mc, GM = G + Mo).  Adjoint: the mass terms are
+            // weighted by conj(wn) at the output and stay apart; F starts
+            // from the pending plane-below sum and Mc from the pending mass
+            // sum, so both need no separate add.
             cpx<R> Qn = zero<R>(), Fn = zero<R>(), Gn = zero<R>();
             if constexpr (!ADJ) {
                 Fn = Mc;
                 Gn = Mo;
+            } else {
+                Fn = Pend[r];
             }
             if constexpr (!DERIV) {
                 const cpx<R> u11 = u[1][r + 1];
@@ -504,7 +508,7 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                     // column's own values, not sums of the side ones.
                     const cpx<R> sy0 = u[0][r] + u[0][r + 2], sy2 = u[2][r] + u[2][r + 2];
                     // accumulated multiplier by multiplier (few live temporaries)
-                    Fn = cmul(px0, rfma(cx * u[0][r + 1], -wxy, sy0));
+                    Fn = cfma(Fn, px0, rfma(cx * u[0][r + 1], -wxy, sy0));
                     Gn = cmul(px0, rfma((-wxz) * u[0][r + 1], -w3, sy0));
                     Fn = cfma(Fn, px2, rfma(cx * u[2][r + 1], -wxy, sy2));
                     Gn = cfma(Gn, px2, rfma((-wxz) * u[2][r + 1], -w3, sy2));
@@ -548,8 +552,8 @@ __device__ __forceinline__ void helm27_body(const StencilLaunch<R>& L, const int
                     out = Acc[r] + Sup;
                     out = cfma(out, conjc(wc[r]), MAcc[r] + Mo);
                 }
-                Acc[r] = Pend[r] + S0;
-                MAcc[r] = MOcur[r] + Mc;
+                Acc[r] = S0;
+                MAcc[r] = Mc;
                 Pend[r] = Sdn;
                 MOcur[r] = Mo;
             }
