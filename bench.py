@@ -733,10 +733,10 @@ def main():
     for r in (roofline, stencil_roofline):
         r["measured_copy_GBps"] = copy_gbs
         r["frac_of_measured_copy"] = r["achieved"] / copy_gbs
-    stencil_roofline["note"] = ("near the FP64 vector ridge (78.6 TFLOP/s / 8 TB/s = 10 flop/B): 120-190 FP64 "
-                                "instructions and 18 LDS reads per point against 40 B; the arithmetic does not "
-                                "overlap memory (a copy of the kernel without it runs at 4.8-5.3 TB/s, and three "
-                                "waves per SIMD instead of two gave the same time)")
+    stencil_roofline["note"] = ("near the FP64 vector ridge (78.6 TFLOP/s / 8 TB/s = 10 flop/B): about 176 VALU "
+                                "instructions (mostly FP64) and 16 LDS instructions per point against 40 B; the "
+                                "arithmetic does not fully overlap memory (a copy of the kernel without it runs at "
+                                "4.8-5.3 TB/s), and more planes of loads in flight cost registers")
     for v in breakdown.values():
         for k in ("alg_bytes", "total_ms", "launches"):
             v.pop(k)
