@@ -3,6 +3,7 @@
 // Host side of the hot path: operator contexts (device-resident model, PML
 // tables, Krylov workspace), the drop-in matvec of the MEX boundary, and the
 // restarted GMRES driver of FGMRES.m with its FP64 Hessenberg least squares.
+#include <algorithm>
 #include <type_traits>
 
 #include "capi_internal.hpp"
@@ -857,11 +858,21 @@ int helm3d_autotune(helm3d_ctx* c, int mode, int precision, int reps, int* kchun
     StencilGrid best = keep;
     double best_t = 1e300;
     int rc = HELM3D_OK;
-    // candidate z chunks: whole column, halves, ... down to 8 planes
+    // candidate z chunks: whole column, halves, ... down to 8 planes.  A grid
+    // of few column tiles (the coarse multigrid levels) also tries chunks of
+    // 4, 2 and 1 planes: its launches are short marches on a few compute
+    // units, and shorter marches on more blocks cut their latency even though
+    // each chunk reads its two halo planes again.
     std::vector<int> chunks;
     for (int d : {1, 2, 3, 4, 6, 8, 12}) {
         const int kc = (c->n[2] + d - 1) / d;
-        if (kc >= 8 || d == 1) chunks.push_back(kc);
+        if ((kc >= 8 || d == 1) && std::find(chunks.begin(), chunks.end(), kc) == chunks.end()) chunks.push_back(kc);
+    }
+    const long tiles = long((c->n[0] + 63) / 64) * ((c->n[1] + 3) / 4);
+    if (tiles < 128) {
+        for (int kc : {4, 2, 1}) {
+            if (kc < c->n[2] && std::find(chunks.begin(), chunks.end(), kc) == chunks.end()) chunks.push_back(kc);
+        }
     }
     // (tile rows, rows per thread): 64 x 4 and 64 x 8 tiles, the latter also
     // with two rows per thread

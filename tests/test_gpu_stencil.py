@@ -115,7 +115,8 @@ def test_device_deriv():
 
 
 SHAPES = [(0, 4, 1, 1), (35, 4, 2, 1), (70, 8, 1, 1), (17, 8, 2, 1), (23, 4, 3, 1), (9, 8, 4, 1), (70, 4, 4, 1),
-          (0, 8, 1, 2), (70, 8, 2, 2), (17, 8, 3, 2), (9, 8, 4, 2)]
+          (0, 8, 1, 2), (70, 8, 2, 2), (17, 8, 3, 2), (9, 8, 4, 2),
+          (1, 4, 2, 1), (1, 8, 4, 1), (2, 4, 3, 1), (2, 8, 2, 2), (4, 4, 1, 1), (1, 8, 1, 2)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
