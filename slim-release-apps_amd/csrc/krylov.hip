@@ -354,7 +354,12 @@ hipError_t dispatch_k(int k, Fn&& fn)
 
 int krylov_blocks(long n)
 {
+    // About 8 points per thread; a vector too short to fill 256 blocks that
+    // way (the coarse multigrid levels) gets up to 256 blocks of one or a few
+    // points per thread, which shortens the per-thread loops of these
+    // latency-bound launches.
     long b = (n + KB * 8 - 1) / (KB * 8);
+    if (b < 256) b = std::min((n + KB - 1) / KB, 256L);
     return int(std::max(1L, std::min(b, long(kMaxPartials))));
 }
 
