@@ -39,7 +39,7 @@ template <typename R>
 hipError_t launch_update(const cpx<R>* V, long ldv, int k, const double* d, const CoefPack& scl, cpx<R>* w,
                          long n, double* part, int nblk, hipStream_t s, const double* nrm = nullptr,
                          const double* dpart = nullptr, double* hrow = nullptr, const int* skip = nullptr);
-constexpr int kInBlockDots = 64;  // dots columns of at most this many partials are summed in the update pass
+constexpr int kInBlockDots = 256;  // dots columns of at most this many partials are summed in the update pass
 template <typename R>
 hipError_t launch_combine(const cpx<R>* V, long ldv, int k, const CoefPack& a, cpx<R>* x, long n, int nblk,
                           hipStream_t s);
