@@ -164,7 +164,8 @@ int helm3d_get_launch(helm3d_ctx *ctx, int *kchunk, int *tile_rows, int *prefetc
 /* Autotuner of the stencil launch shape for this context's grid: times every
  * candidate shape (64 x 4 and 64 x 8 column tiles, the latter with one or two
  * rows per thread, prefetch {1, 2, 3, 4}, z chunks of the whole column down
- * to 8 planes) on the operator of `mode` (1: H, else H^H) in `precision`,
+ * to 8 planes, and of 4, 2 and 1 planes on grids of fewer than 128 64 x 4
+ * column tiles) on the operator of `mode` (1: H, else H^H) in `precision`,
  * `reps` launches each with HIP events on the context's stream, and keeps the
  * fastest (results are identical for every shape).  No counterpart in the
  * reference; the batched solves use each item's tuned z chunk and the first
