@@ -20,6 +20,35 @@ def _setup(cfg):
     return wn, b
 
 
+@pytest.mark.parametrize("n", [(5, 4, 4), (3, 1, 2), (1, 1, 1)])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_solve_tiny_grids(n, mode):
+    """FGMRES(20) on grids of 1 to 80 points: one reduction block, vectors
+    shorter than a block's threads, and (3 x 1 x 2, 1 x 1 x 1) a Krylov space
+    exhausted before the tolerance is met (the step after is a breakdown)."""
+    import torch
+    from helm3d import Helm3d
+    N = int(np.prod(n))
+    rng = np.random.default_rng(5)
+    wn = rng.uniform(2e-4, 2e-3, N)
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    npml = np.array([[1, 1, 1], [1, 1, 1]])
+    h = (10.0, 10.0, 10.0)
+    variant = "forw" if mode == 1 else "adj"
+    print(f"tiny solve n={n} mode={mode}", flush=True)
+    xo, info_o = O.fgmres(variant, wn, h, n, npml, b, tol=1e-6, maxinnerit=20)
+    dev = torch.device("cuda:0")
+    H = Helm3d(n, h, npml, torch.from_numpy(wn).to(dev))
+    x, info = H.solve(torch.from_numpy(b).to(dev), mode=mode, tol=1e-6, maxinnerit=20)
+    x = x.cpu().numpy()
+    assert np.isfinite(x).all()
+    relres = np.linalg.norm(b - O.hmvp(variant, wn, h, n, npml, x)) / np.linalg.norm(b)
+    print("gpu iters", info.iters, "oracle iters", info_o["iters"], "relres", relres, flush=True)
+    assert info.converged and relres < 1e-6
+    assert abs(info.iters - info_o["iters"]) <= 0.02 * info_o["iters"] + 2
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-3
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_solve_matches_oracle(mode):
     import torch
