@@ -57,6 +57,31 @@ def test_device_sweep_matches_oracle(case, variant):
     assert relerr(y2, yo2) <= 1e-12
 
 
+# grids with an axis of one or two points: colours without rows on an axis
+# are skipped, every row sits on the grid edge
+DEGENERATE = [
+    ((5, 1, 4), [[1, 1, 1], [1, 1, 1]], (10.0, 10.0, 10.0)),
+    ((2, 2, 2), [[1, 1, 1], [1, 1, 1]], (10.0, 10.0, 10.0)),
+    ((1, 1, 1), [[1, 1, 1], [1, 1, 1]], (10.0, 10.0, 10.0)),
+    ((4, 3, 1), [[1, 1, 0], [1, 1, 1]], (10.0, 10.0, 10.0)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(DEGENERATE)))
+@pytest.mark.parametrize("variant", ["forw", "adj"])
+def test_device_sweep_degenerate_grids(case, variant):
+    import torch
+    from helm3d import Helm3d
+    n, npml, h = DEGENERATE[case]
+    wn, x, b = _inputs(n, 80 + case, False)
+    print(f"degenerate sweep case={case} n={n} variant={variant}", flush=True)
+    H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).cuda())
+    mode = 2 if variant == "adj" else 1
+    y = H.kacz_sweep(torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda(), mode=mode, omega=1.5).cpu().numpy()
+    yo = O.kacz_sweep(variant, wn, h, n, np.array(npml), 1.5, x, b, order="mc")
+    assert relerr(y, yo) <= 1e-12
+
+
 def test_dropin_gateway_and_errors():
     from helm3d import Helm3dError, helm3d_kaczswp_host
     n, npml, h = CASES[1]
