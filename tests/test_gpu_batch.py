@@ -72,6 +72,34 @@ def test_plain_lockstep_uneven_grids(mode):
     assert len({inf.iters for inf in infos}) == 1  # lockstep (FGMRES_MT.m)
 
 
+@pytest.mark.parametrize("method", ["plain", "mixed"])
+def test_batch_with_degenerate_item(method):
+    """A batch holding a grid of 3 x 1 x 2 points, whose Krylov space runs out
+    before the tolerance (a breakdown ends its cycles), beside a 48 x 48 x 40
+    grid: both items converge and the small one stays finite."""
+    import torch
+    from helm3d import BatchSolver, Helm3d
+    c1, w1, H1 = _op((32, 32, 24), 8, 5.0, 12)
+    b1 = _rhs(c1, (16, 16, 3))
+    n2, p2, h2 = (3, 1, 2), np.array([[1, 1, 1], [1, 1, 1]]), (10.0, 10.0, 10.0)
+    rng = np.random.default_rng(21)
+    w2 = rng.uniform(2e-4, 2e-3, 6)
+    b2 = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+    H2 = Helm3d(n2, h2, p2, torch.from_numpy(w2).to("cuda:0"))
+    print("batch with a degenerate item", method, flush=True)
+    S = BatchSolver("cuda:0")
+    extra = {"inner_maxit": 10} if method == "mixed" else {}
+    xs, infos = S.solve([H1, H2], [torch.from_numpy(b1).cuda(), torch.from_numpy(b2).cuda()], mode=1,
+                        method=method, tol=1e-6, **extra)
+    r1 = _relres(c1, w1, b1, xs[0].cpu().numpy(), 1)
+    x2 = xs[1].cpu().numpy()
+    assert np.isfinite(x2).all()
+    r2 = np.linalg.norm(b2 - O.hmvp("forw", w2, h2, n2, p2, x2)) / np.linalg.norm(b2)
+    print("items", [(i.iters, i.cycles, i.relres) for i in infos], r1, r2, flush=True)
+    assert infos[0].converged and infos[1].converged
+    assert r1 < 1e-6 and r2 < 1e-6
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_mixed_batch_uneven_grids(mode):
     import torch
