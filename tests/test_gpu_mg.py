@@ -88,6 +88,61 @@ def test_mg_solve_cycle_sizes(k):
     M.close()
 
 
+# (n_phys, npml, levels, free surface, k) and the grids each hierarchy builds;
+# every coarsened level keeps the 4 physical points per axis that the cubic
+# model coarsening needs
+TINY = [
+    ((7, 8, 7), 1, 3, False, (1, 2, 3, 5)),       # 9x10x9, 6x6x6, 4x4x4
+    ((13, 13, 13), 2, 4, True, (3, 5, 3, 5)),     # 17x17x15, 9x9x8, 6x6x5, 4x4x3
+    ((4, 5, 4), 1, 2, False, (1, 30, 2, 30)),     # 6x7x6, 4x5x4: 30-step cycles on 80 to 252 points
+]
+
+
+@pytest.mark.parametrize("case", range(len(TINY)))
+def test_tiny_hierarchies(case):
+    """The smallest hierarchies the coarsening allows, down to 3 or 4 points
+    per axis with the PML: the transfers against the oracle's P, and the
+    V-cycle solve in both precisions and directions (device-driven cycles of
+    up to 30 steps on vectors of 48 to 4335 points, shorter than one block of
+    threads on the coarsest levels) reaching the tolerance with the oracle's
+    operator and the oracle's solution (condition numbers 10 to 84, so the
+    1e-6 residual bounds the error by about 1e-4)."""
+    import torch
+    from helm3d.mg import MLGMRES
+    n, npml, nl, fs, k = TINY[case]
+    rng = np.random.default_rng(40 + case)
+    v = 2000.0 + 200.0 * rng.standard_normal(int(np.prod(n)))
+    M = MLGMRES(v, n, 10.0, npml, 6.0, nlevels=nl, k=k, free_surface=fs)
+    print(f"tiny hierarchy {[g['nt'] for g in M.grids]} k={k}", flush=True)
+    for lvl in range(nl - 1):
+        nf, nc = M.grids[lvl]["nt"], M.grids[lvl + 1]["nt"]
+        P = O.prolongation(nf, nc)
+        xc = rng.standard_normal(P.shape[1]) + 1j * rng.standard_normal(P.shape[1])
+        xf = rng.standard_normal(P.shape[0]) + 1j * rng.standard_normal(P.shape[0])
+        yf = M.transfer(lvl, torch.from_numpy(xc).cuda(), "prolong").cpu().numpy()
+        yc = M.transfer(lvl, torch.from_numpy(xf).cuda(), "restrict").cpu().numpy()
+        assert np.linalg.norm(yf - P @ xc) <= 1e-13 * np.linalg.norm(P @ xc)
+        assert np.linalg.norm(yc - (P.T @ xf) / 8) <= 1e-13 * np.linalg.norm(P.T @ xf / 8)
+    wn = M.H.wn.cpu().numpy()
+    nt, p6, h3 = M.grids[0]["nt"], np.array(M.grids[0]["p6"]), (10.0,) * 3
+    N = int(np.prod(nt))
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    for mode in (1, 2):
+        variant = "forw" if mode == 1 else "adj"
+        xo, _ = O.fgmres(variant, wn, h3, nt, p6, b, tol=1e-10, maxinnerit=20)
+        for precision in ("c128", "mixed"):
+            x, info = M.solve(torch.from_numpy(b).cuda(), mode=mode, tol=1e-6, maxit=200, precision=precision)
+            xn = x.cpu().numpy()
+            assert np.isfinite(xn).all()
+            rel = np.linalg.norm(b - O.hmvp(variant, wn, h3, nt, p6, xn)) / np.linalg.norm(b)
+            err = np.linalg.norm(xn - xo) / np.linalg.norm(xo)
+            print(f"  mode={mode} {precision}: iters={info.iters} relres={info.relres:.2e} (oracle {rel:.2e}) "
+                  f"err={err:.2e}", flush=True)
+            assert info.converged and rel < 1e-6 and abs(rel - info.relres) < 1e-9
+            assert err < 5e-4
+    M.close()
+
+
 def test_pool_of_hierarchies_matches_single():
     """Right-hand sides solved three at a time on separate streams (MGPool)
     give the solutions of one-at-a-time solves (to rounding: the hierarchies'
