@@ -82,6 +82,36 @@ def test_device_sweep_degenerate_grids(case, variant):
     assert relerr(y, yo) <= 1e-12
 
 
+@pytest.mark.parametrize("case", range(len(DEGENERATE)))
+@pytest.mark.parametrize("solver", ["cgmn", "crmn"])
+def test_solvers_degenerate_grids(case, solver):
+    """CGMN and CRMN on grids with an axis of one or two points, forward and
+    adjoint: the oracle's iteration count (1 to 10 here; on 1 x 1 x 1 the
+    first step can leave an exactly zero residual) and its solution."""
+    import torch
+    from helm3d import Helm3d
+    n, npml, h = DEGENERATE[case]
+    rng = np.random.default_rng(90 + case)
+    N = int(np.prod(n))
+    wn = rng.uniform(2e-4, 2e-3, N)
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    H = Helm3d(n, h, np.array(npml), torch.from_numpy(wn).cuda())
+    for mode, v in ((1, "forw"), (2, "adj")):
+        print(f"degenerate {solver} case={case} n={n} mode={mode}", flush=True)
+        fn = H.solve_cgmn if solver == "cgmn" else H.solve_crmn
+        x, info = fn(torch.from_numpy(b).cuda(), mode=mode, tol=1e-8, maxit=500)
+        x = x.cpu().numpy()
+        sweep = lambda y, r: O.kacz_sweep(v, wn, h, n, np.array(npml), 1.5, y, r)  # noqa: E731
+        xo, its, hst = (O.cgmn if solver == "cgmn" else O.crmn)(sweep, b, tol=1e-8, maxit=500)
+        print(f"  gpu {info.iters} {info.quasi_relres:.1e} oracle {its} {hst[-1]:.1e}", flush=True)
+        assert np.isfinite(x).all()
+        assert info.converged and info.quasi_relres <= 1e-8
+        assert abs(info.iters - its) <= 0.02 * its + 2
+        assert relerr(x, xo) <= 1e-6
+        rel = np.linalg.norm(b - O.hmvp(v, wn, h, n, np.array(npml), x)) / np.linalg.norm(b)
+        assert abs(rel - info.relres) <= 1e-9 + 1e-6 * rel
+
+
 def test_dropin_gateway_and_errors():
     from helm3d import Helm3dError, helm3d_kaczswp_host
     n, npml, h = CASES[1]

@@ -49,6 +49,49 @@ def test_solve_tiny_grids(n, mode):
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-3
 
 
+@pytest.mark.parametrize("n", [(5, 4, 4), (3, 1, 2), (1, 1, 1)])
+@pytest.mark.parametrize("mode", [1, 2])
+def test_bicgstab_and_mixed_tiny_grids(n, mode):
+    """FP64 BiCGStab and the mixed-precision solve (complex64 BiCGStab to a
+    tolerance, complex64 GMRES, and the fixed-length BiCGStab replayed from a
+    graph) on grids of 1 to 80 points.  On 1 x 1 x 1 one iteration solves the
+    system, so the fixed-length inner solve runs on with zero or rounding-
+    sized residuals, where its scalar kernels' zero-denominator guards give
+    zero steps."""
+    import torch
+    from helm3d import Helm3d
+    N = int(np.prod(n))
+    rng = np.random.default_rng(5)
+    wn = rng.uniform(2e-4, 2e-3, N)
+    b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    npml = np.array([[1, 1, 1], [1, 1, 1]])
+    h = (10.0, 10.0, 10.0)
+    variant = "forw" if mode == 1 else "adj"
+    dev = torch.device("cuda:0")
+    H = Helm3d(n, h, npml, torch.from_numpy(wn).to(dev))
+    bt = torch.from_numpy(b).to(dev)
+    xf, _ = O.fgmres(variant, wn, h, n, npml, b, tol=1e-12, maxinnerit=20)
+
+    def check(x, info, what):
+        x = x.cpu().numpy()
+        rel = np.linalg.norm(b - O.hmvp(variant, wn, h, n, npml, x)) / np.linalg.norm(b)
+        err = np.linalg.norm(x - xf) / np.linalg.norm(xf)
+        print(f"  {what}: iters={info.iters} relres={info.relres:.1e} (oracle {rel:.1e}) err={err:.1e}", flush=True)
+        assert np.isfinite(x).all() and info.converged
+        assert rel < 1e-6 and abs(rel - info.relres) < 1e-9
+        assert err < 1e-3
+
+    print(f"tiny bicgstab/mixed n={n} mode={mode}", flush=True)
+    with np.errstate(all="ignore"):
+        _, it_o, _ = O.bicgstab(variant, wn, h, n, npml, b, tol=1e-6, maxit=200)
+    x, info = H.solve_bicgstab(bt, mode=mode, tol=1e-6, maxit=200)
+    check(x, info, f"bicgstab fp64 (oracle {it_o} iterations)")
+    assert 0.75 * it_o <= info.iters // 2 <= 1.33 * it_o + 2
+    for inner, itol in (("bicgstab", 1e-2), ("gmres", 1e-2), ("bicgstab", 0.0)):
+        x, info = H.solve_mixed(bt, mode=mode, tol=1e-6, maxit=200, inner=inner, inner_tol=itol, inner_maxit=10)
+        check(x, info, f"mixed {inner} inner_tol={itol}")
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_solve_matches_oracle(mode):
     import torch
