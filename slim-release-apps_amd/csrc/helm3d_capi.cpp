@@ -221,6 +221,11 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
         Z = Zb.p;
     }
     const int nblk = krylov_blocks(N);
+    // Gram-Schmidt passes: their own block counts on long complex128 vectors
+    // (equal to nblk on short ones; complex64 keeps nblk)
+    constexpr bool kWide = std::is_same<R, double>::value;
+    const int nblk_d = kWide ? dots_blocks(N) : nblk;
+    const int nblk_u = kWide ? update_blocks(N) : nblk;
     auto slot = [&](int s) { return V + size_t(s) * ld; };
     GmresDevState* st = c->gst;
     const int* skip = device_driven ? &st->skip : nullptr;
@@ -258,12 +263,13 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, nullptr, CoefPack{}, slot(j), N, c->part,
                                                        nblk, s, c->nrm, c->part + nblk, hrow, skip));
         } else {
-            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk, s, skip));
-            TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * j, row, s, hrow, skip));
-            TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk, s,
+            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, j, slot(j), N, c->part, nblk_d, s, skip));
+            TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk_d, 2 * j, row, s, hrow, skip));
+            TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, j, row, CoefPack{}, slot(j), N, c->part, nblk_u, s,
                                                        c->nrm, nullptr, nullptr, skip));
         }
-        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 1, c->nrm + j, s, hrow + 2 * j, skip));
+        const int nu = nblk <= kInBlockDots ? nblk : nblk_u;
+        TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nu, 1, c->nrm + j, s, hrow + 2 * j, skip));
         if (record) HIPCHK(hipEventRecord(c->ring_ev[j], s));
         return HELM3D_OK;
     };
@@ -392,11 +398,12 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             // ||w||^2 reach c->small and its pinned mirror
             CoefPack scl{};
             for (int q = 0; q < k; ++q) scl.v[q] = sig[q] * sig[q];
-            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk, c->stream));
-            TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk, 2 * k, c->small, c->stream, c->small_host_dev));
-            TIMED(HELM3D_PROF_UPDATE, launch_update<R>(V, ld, k, c->small, scl, slot(k), N, c->part, nblk, c->stream));
+            TIMED(HELM3D_PROF_DOTS, launch_dots<R>(V, ld, k, slot(k), N, c->part, nblk_d, c->stream));
+            TIMED(HELM3D_PROF_REDUCE, launch_reduce(c->part, nblk_d, 2 * k, c->small, c->stream, c->small_host_dev));
+            TIMED(HELM3D_PROF_UPDATE,
+                  launch_update<R>(V, ld, k, c->small, scl, slot(k), N, c->part, nblk_u, c->stream));
             TIMED(HELM3D_PROF_REDUCE,
-                  launch_reduce(c->part, nblk, 1, c->small + 2 * k, c->stream, c->small_host_dev + 2 * k));
+                  launch_reduce(c->part, nblk_u, 1, c->small + 2 * k, c->stream, c->small_host_dev + 2 * k));
             HIPCHK(hipStreamSynchronize(c->stream));
             if (c->prof) CHK(prof_fold(c));
             for (int q = 0; q < k; ++q)

@@ -363,6 +363,22 @@ int krylov_blocks(long n)
     return int(std::max(1L, std::min(b, long(kMaxPartials))));
 }
 
+int dots_blocks(long n)
+{
+    const long kb = krylov_blocks(n);
+    if (kb <= kInBlockDots) return int(kb);
+    const long b = std::max((n + KB * 32 - 1) / (KB * 32), std::min(kb, 512L));
+    return int(std::min(b, long(kMaxPartials)));
+}
+
+int update_blocks(long n)
+{
+    const long kb = krylov_blocks(n);
+    if (kb <= kInBlockDots) return int(kb);
+    const long b = std::max((n + KB * 2 - 1) / (KB * 2), kb);
+    return int(std::min(b, long(kMaxUpdateBlocks)));
+}
+
 // Streaming basis loads once the basis slots read by a pass exceed the
 // last-level cache (256 MB Infinity Cache on MI355X); smaller bases (coarse
 // multigrid levels, small grids) stay cacheable between the dots and update
