@@ -218,8 +218,19 @@ __device__ __forceinline__ void norm2_body(const cpx<R>* __restrict__ x, long n,
 __device__ __forceinline__ double reduce_body(const double* __restrict__ part, int nblk, int q)
 {
     __shared__ double red[KB];
+    const double* __restrict__ p = part + long(q) * nblk;
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += KB) s += part[long(q) * nblk + b];
+    int b = threadIdx.x;
+    // eight loads in flight per trip, added in the same order as one at a
+    // time (long columns: the update pass leaves up to 16384 partials)
+    for (; b + 7 * KB < nblk; b += 8 * KB) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = p[b + u * KB];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; b < nblk; b += KB) s += p[b];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int o = KB / 2; o > 0; o >>= 1) {
