@@ -222,7 +222,8 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
     }
     const int nblk = krylov_blocks(N);
     // Gram-Schmidt passes: their own block counts on long complex128 vectors
-    // (equal to nblk on short ones; complex64 keeps nblk)
+    // (equal to nblk on short ones; complex64 keeps nblk); the solution update
+    // x += V y streams like the update pass and takes its count
     constexpr bool kWide = std::is_same<R, double>::value;
     const int nblk_d = kWide ? dots_blocks(N) : nblk;
     const int nblk_u = kWide ? update_blocks(N) : nblk;
@@ -305,7 +306,7 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             }
             CHK(enqueue_cycle(use_graph));
             HIPCHK(launch_gmres_lsq(c->gs, kGsRow, m, tol, st, c->stream));
-            TIMED(HELM3D_PROF_COMBINE, launch_combine_dev<R>(V, ld, m, st->coef, &st->kept, x, N, nblk, c->stream, skip));
+            TIMED(HELM3D_PROF_COMBINE, launch_combine_dev<R>(V, ld, m, st->coef, &st->kept, x, N, nblk_u, c->stream, skip));
         }
         // no host copy of the outcome: the caller uses x only
         res->iters = -1;
@@ -429,9 +430,9 @@ int gmres_engine(helm3d_ctx* c, int variant, const cpx<R>* b, cpx<R>* x, bool us
             a.v[2 * q + 1] = v.imag();
         }
         if (P) {
-            TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(Z, ld, k - 1, a, x, N, nblk, c->stream));
+            TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(Z, ld, k - 1, a, x, N, nblk_u, c->stream));
         } else {
-            TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(V, ld, k - 1, a, x, N, nblk, c->stream));
+            TIMED(HELM3D_PROF_COMBINE, launch_combine<R>(V, ld, k - 1, a, x, N, nblk_u, c->stream));
         }
         // no further cycle (quasi-residual below tol, maxit or the iteration
         // cap): the true residual decides nothing, so a caller that uses only
